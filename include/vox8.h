@@ -1,0 +1,142 @@
+/* vox8 — C ABI of the MI355X-native voxel hex8 FEM hot path.
+ *
+ * Plain C types only: device buffers are raw pointers (allocated by the caller,
+ * e.g. torch tensors), host buffers are marked h_, device buffers d_.  Every
+ * call returns 0 on success and a negative status on error; vox8_last_error()
+ * then holds a message.  Solver calls report non-convergence through `info`
+ * (scipy convention: 0 converged, >0 iterations done without converging), not
+ * through the return status.  A context is bound to one HIP stream and is not
+ * thread-safe; separate contexts are independent.
+ *
+ * Grid layout: nodes n = ix + (nx+1)(iy + (ny+1) iz), DOF 3n+c, elements
+ * e = ex + nx(ey + ny ez); blocks of vectors are column-major with leading
+ * dimension `ld` (>= 3 * n_nodes).
+ *
+ * The reference (NinhGhoster/TO-F450-arm) is NumPy/SciPy with a CuPy
+ * monkey-patch; each entry point names the reference function it replaces.
+ */
+#ifndef VOX8_H
+#define VOX8_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vox8_ctx vox8_ctx;
+
+enum { VOX8_OP_K = 1, VOX8_OP_M = 2, VOX8_OP_KM = 3 };
+enum { VOX8_PREC_FP64 = 0, VOX8_PREC_MIXED = 1, VOX8_PREC_FP32 = 2 };
+
+/* Last error message of the calling thread ("" if none). */
+const char* vox8_last_error(void);
+/* ABI version (major*100 + minor). */
+int vox8_version(void);
+/* Number of visible HIP devices (replaces cp.cuda.runtime.getDeviceCount,
+ * src/fem_gpu.py:37, 42-43). */
+int vox8_device_count(int* h_count);
+
+/* Context for one nx x ny x nz voxel box on the current device, bound to
+ * `hip_stream` (NULL = default stream).  Replaces FEM3D.__init__'s
+ * per-problem state (src/fem.py:207-308); no COO/CSR pattern is built. */
+int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream);
+int vox8_destroy(vox8_ctx* ctx);
+int vox8_set_stream(vox8_ctx* ctx, void* hip_stream);
+
+/* Unit-density element matrices, host 24x24 row-major, DOF order
+ * [n0x,n0y,n0z,n1x,...] in the reference local node order.  Replaces
+ * FEM3D.Ke_solid / Me_solid (src/fem.py:211-221); call again when a caller
+ * mutates Ke_solid (anisotropy study).  Selects the structured stencil when Ke
+ * has the box-element block structure, else the general 24x24 stencil. */
+int vox8_set_element_matrices(vox8_ctx* ctx, const double* h_Ke, const double* h_Me);
+
+/* Active elements (h_elem_active[n_elem], 1 = in the model) and the fixed-DOF
+ * bitmask per node (bit c = DOF 3n+c fixed; ghost nodes must be fixed).
+ * Replaces the masks / fixed_dofs / ghost-DOF logic of src/fem.py:225-305. */
+int vox8_set_topology(vox8_ctx* ctx, const uint8_t* h_elem_active, const uint8_t* h_node_fixed);
+
+/* Lumped tip mass: mass_kg / n_nodes on the three translational DOFs of every
+ * listed node (src/fem.py:383-398).  n_nodes = 0 clears it. */
+int vox8_set_tip_mass(vox8_ctx* ctx, const int64_t* h_nodes, int n_nodes, double mass_kg);
+
+/* Element densities d_rho[n_elem] (element grid order).  Computes the SIMP
+ * factor e_min + (1-e_min) rho^p and the Pedersen mass factor on the device
+ * (src/fem.py:316-320 assemble_K, 334-349 _mass_simp_factor). */
+int vox8_set_density(vox8_ctx* ctx, const double* d_rho, double p, double e_min,
+                     double mass_threshold, double mass_q);
+
+/* Y = K X and/or M X on the free DOFs for nrhs column-major vectors (X read as
+ * zero at fixed DOFs, Y zero there).  prec FP64: double buffers; FP32: float
+ * buffers.  Replaces assemble_K/assemble_M + CSR SpMV (src/fem.py:311-398,
+ * K_ff @ x in scipy cg / lobpcg). */
+int vox8_apply(vox8_ctx* ctx, int op, int prec, const void* d_X, void* d_YK, void* d_YM,
+               int nrhs, int64_t ld);
+
+/* Diagonal of K_ff or M_ff (zeros at fixed DOFs), K_ff.diagonal() of
+ * src/fem.py:441,543. */
+int vox8_diagonal(vox8_ctx* ctx, int op, double* d_diag);
+
+/* Jacobi-PCG on K_ff U = F_f for nrhs load cases at once (independent CGs
+ * sharing every launch).  d_U holds x0 when x0_given, else is overwritten.
+ * Semantics of scipy.sparse.linalg.cg as called at src/fem.py:543-547 and
+ * src/fem_gpu.py:61-81: stop when ||r|| < rtol ||F_f||, maxiter -> info.
+ * prec MIXED: FP32 inner PCG inside FP64 iterative refinement on the FP64
+ * residual (same stop test).  h_iters/h_info/h_relres: arrays of nrhs. */
+int vox8_pcg(vox8_ctx* ctx, const double* d_F, double* d_U, int nrhs, int64_t ld, int x0_given,
+             double rtol, int maxiter, int prec, int* h_iters, int* h_info, double* h_relres);
+
+/* k smallest eigenpairs of K_ff x = lam M_ff x (M includes the tip lump) by
+ * Jacobi-preconditioned LOBPCG; replaces scipy.sparse.linalg.lobpcg at
+ * src/fem.py:450-454 and src/fem_gpu.py:149-152.  d_X0 (k column-major
+ * vectors, may be NULL for a seeded random block), d_X receives M-orthonormal
+ * modes; h_lam[k] ascending.  Stop: every column's absolute residual
+ * ||K x - lam M x||_2 < tol (scipy semantics). */
+int vox8_lobpcg(vox8_ctx* ctx, int k, double tol, int maxiter, int prec, const double* d_X0,
+                double* d_X, double* h_lam, int* h_iters, int* h_info, double* h_max_res,
+                uint64_t seed);
+
+/* Adjoint quantities for the current density:
+ *   d_ce[e]  = sum_j w_j U_j,e^T Ke U_j,e      (elem_compliance, src/fem.py:605-615)
+ *   d_dc[e]  = -p (1-e_min) rho^(p-1) d_ce[e]  (src/topopt.py:296-306)
+ *   h_comp[j]= sum_e f_e U_j,e^T Ke U_j,e      (per-load-case compliance)
+ * and, when d_phi != NULL, the eigenvalue sensitivity of (lam, phi)
+ *   d_dlam[e] = ((1-e_min) p max(rho,1e-9)^(p-1) phi_e^T Ke phi_e
+ *               - lam m'(rho) phi_e^T Me phi_e) / phi^T M phi   (src/fem.py:472-526)
+ * with h_vmv = phi^T M phi (tip lump included).  Output pointers may be NULL. */
+int vox8_sensitivities(vox8_ctx* ctx, const double* d_U, int nrhs, int64_t ld,
+                       const double* h_w, const double* d_phi, double lam, const double* d_rho,
+                       double p, double e_min, double mass_threshold, double mass_q,
+                       double* d_ce, double* d_dc, double* d_dlam, double* h_comp,
+                       double* h_vmv);
+
+/* Cone density filter w(d) = max(0, r - d) on the element grid restricted to
+ * d_mask (cone_kernel / filter_field, src/topopt.py:32-58).  setup builds the
+ * taps and the masked denominator; filter applies it (d_in, d_out n_elem). */
+int vox8_filter_setup(vox8_ctx* ctx, double r_min_vox, const uint8_t* d_mask);
+int vox8_filter(vox8_ctx* ctx, const double* d_in, double* d_out);
+
+/* OC update with bisection on the multiplier (src/topopt.py:190-214) over
+ * the elements with d_design[e] != 0; other entries of d_xnew copy d_x. */
+int vox8_oc_update(vox8_ctx* ctx, const uint8_t* d_design, const double* d_x, const double* d_dc,
+                   double V_e, double target_volume, double move, double* d_xnew,
+                   double* h_lmid);
+
+/* Loop bookkeeping reductions (src/topopt.py:351-354, 383-397, 178-188):
+ * h_out4 = {sum a, sum |a|, max |a-b|, count} over d_mask (NULL = all). */
+int vox8_stats(vox8_ctx* ctx, int64_t n, const uint8_t* d_mask, const double* d_a,
+               const double* d_b, double* h_out4);
+/* h_out[2k+1] = {X_j.v (k), X_j.X_j (k), v.v} for k column-major vectors. */
+int vox8_block_dots(vox8_ctx* ctx, int64_t n, int k, const double* d_X, int64_t ld,
+                    const double* d_v, double* h_out);
+/* y = a x + b y on d_mask (NULL = all). */
+int vox8_axpby(vox8_ctx* ctx, int64_t n, const uint8_t* d_mask, double a, const double* d_x,
+               double b, double* d_y);
+/* out = design ? filt : (in_arm ? 1 : 0)  (src/topopt.py:273-281). */
+int vox8_compose_density(vox8_ctx* ctx, const uint8_t* d_design, const uint8_t* d_in_arm,
+                         const double* d_filt, double* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOX8_H */

@@ -1,0 +1,549 @@
+// vox8 — C-ABI entry points (declared in include/vox8.h) and the context object.
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "../../include/vox8.h"
+#include "vox8_kernels.h"
+#include "vox8_pcg.h"
+#include "vox8_topopt.h"
+
+namespace vox8 {
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+}  // namespace vox8
+
+using namespace vox8;
+
+struct vox8_ctx {
+  Grid g{};
+  hipStream_t st = nullptr;
+  int device = 0;
+  std::vector<void*> dev_allocs;
+  std::vector<void*> host_allocs;
+  double Ke[576]{}, Me[576]{};
+  bool have_mats = false, have_topo = false, have_density = false;
+  double *dKe = nullptr, *dMe = nullptr;
+  float *dKe32 = nullptr, *dMe32 = nullptr;
+  uint8_t *d_active = nullptr, *d_fixed = nullptr;
+  double *fK = nullptr, *fM = nullptr;
+  float *fK32 = nullptr, *fM32 = nullptr;
+  int64_t* d_tip = nullptr;
+  int n_tip = 0, tip_cap = 0;
+  double tip_m = 0.0;
+  double* dinvK = nullptr;
+  float* dinvK32 = nullptr;
+  bool dinv_valid = false;
+  int dinv_guard = -1;
+  // reductions
+  double* red_partials = nullptr;   // 4096 * 40 doubles
+  unsigned int* red_counters = nullptr;  // 64
+  double* red_out = nullptr;        // 64 doubles
+  double* host = nullptr;           // pinned 64 doubles
+  // PCG
+  PcgWork<double> w64;
+  PcgWork<float> w32;
+  // LOBPCG
+  LobpcgWork lw;
+  // filter
+  int* d_toff = nullptr;
+  double* d_tw = nullptr;
+  int ntap = 0;
+  uint8_t* d_fmask = nullptr;
+  double* d_den = nullptr;
+  bool have_filter = false;
+  // OC / sensitivities scratch
+  double* d_oc = nullptr;
+  double *d_kphi = nullptr, *d_mphi = nullptr, *d_ce = nullptr, *d_sums = nullptr,
+         *d_wts = nullptr;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(vox8_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  VOX8_HIP(hipMalloc(&q, count * sizeof(T) + 256));
+  c->dev_allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return OK;
+}
+template <typename T>
+int halloc(vox8_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  VOX8_HIP(hipHostMalloc(&q, count * sizeof(T) + 64, hipHostMallocDefault));
+  c->host_allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return OK;
+}
+void dfree(vox8_ctx* c, void* p) {
+  if (!p) return;
+  for (auto it = c->dev_allocs.begin(); it != c->dev_allocs.end(); ++it)
+    if (*it == p) { (void)hipFree(p); c->dev_allocs.erase(it); return; }
+}
+void hfree(vox8_ctx* c, void* p) {
+  if (!p) return;
+  for (auto it = c->host_allocs.begin(); it != c->host_allocs.end(); ++it)
+    if (*it == p) { (void)hipHostFree(p); c->host_allocs.erase(it); return; }
+}
+
+int require(vox8_ctx* c, bool density) {
+  VOX8_CHECK(c != nullptr, "null context");
+  if (!c->have_mats) { set_error("element matrices not set (vox8_set_element_matrices)"); return ERR_STATE; }
+  if (!c->have_topo) { set_error("topology not set (vox8_set_topology)"); return ERR_STATE; }
+  if (density && !c->have_density) { set_error("density not set (vox8_set_density)"); return ERR_STATE; }
+  VOX8_HIP(hipSetDevice(c->device));
+  return OK;
+}
+
+int ensure_dinv(vox8_ctx* c, int guard) {
+  if (c->dinv_valid && c->dinv_guard == guard) return OK;
+  VOX8_TRY(launch_jacobi(c->g, c->fK, c->dKe, c->d_fixed, guard, nullptr, 0, 0.0, nullptr, c->dinvK,
+                         c->st));
+  VOX8_TRY(convert_to_float(c->dinvK, c->dinvK32, c->g.n_dofs, c->st));
+  c->dinv_valid = true;
+  c->dinv_guard = guard;
+  return OK;
+}
+
+template <typename T>
+int apply_op(vox8_ctx* c, int op, const T* x, int64_t ldx, T* yK, T* yM, int64_t ldy, int nrhs,
+             DotFuse fuse) {
+  const T* fK = nullptr;
+  const T* fM = nullptr;
+  const T* Ke = nullptr;
+  const T* Me = nullptr;
+  if constexpr (sizeof(T) == 8) {
+    fK = (const T*)c->fK; fM = (const T*)c->fM; Ke = (const T*)c->dKe; Me = (const T*)c->dMe;
+  } else {
+    fK = (const T*)c->fK32; fM = (const T*)c->fM32; Ke = (const T*)c->dKe32; Me = (const T*)c->dMe32;
+  }
+  VOX8_TRY(launch_apply_dense<T>(c->g, op, fK, fM, Ke, Me, c->d_fixed, x, ldx, yK, yM, ldy, nrhs,
+                                 c->st, fuse));
+  if ((op & OP_M) && c->n_tip > 0)
+    VOX8_TRY(launch_tip_mass<T>(c->d_tip, c->n_tip, c->tip_m, c->d_fixed, x, ldx, yM, ldy, nrhs, c->st));
+  return OK;
+}
+
+template <typename T>
+int ensure_pcg_work(vox8_ctx* c, PcgWork<T>& w, int ncols) {
+  if (w.ncols >= ncols) return OK;
+  dfree(c, w.r); dfree(c, w.p); dfree(c, w.q); dfree(c, w.x); dfree(c, w.rhs64);
+  dfree(c, w.scal); dfree(c, w.partials); dfree(c, w.counters);
+  hfree(c, w.host_scal); hfree(c, w.host_scale);
+  const int64_t n = c->g.n_dofs;
+  VOX8_TRY(dalloc(c, &w.r, (size_t)ncols * n));
+  VOX8_TRY(dalloc(c, &w.p, (size_t)ncols * n));
+  VOX8_TRY(dalloc(c, &w.q, (size_t)ncols * n));
+  VOX8_TRY(dalloc(c, &w.x, (size_t)ncols * n));
+  VOX8_TRY(dalloc(c, &w.rhs64, (size_t)ncols * n + ncols));
+  VOX8_TRY(dalloc(c, &w.scal, ncols));
+  const int64_t apply_blocks = (c->g.n_nodes + 255) / 256;
+  const int64_t per_col = apply_blocks > 3 * 1024 ? apply_blocks : 3 * 1024;
+  VOX8_TRY(dalloc(c, &w.partials, (size_t)(ncols * per_col)));
+  VOX8_TRY(dalloc(c, &w.counters, ncols));
+  VOX8_HIP(hipMemset(w.counters, 0, sizeof(unsigned int) * ncols));
+  VOX8_TRY(halloc(c, &w.host_scal, ncols));
+  VOX8_TRY(halloc(c, &w.host_scale, ncols));
+  w.ncols = ncols;
+  w.n = n;
+  return OK;
+}
+
+int ensure_lobpcg_work(vox8_ctx* c, int k) {
+  LobpcgWork& w = c->lw;
+  if (w.kmax >= k) return OK;
+  dfree(c, w.S); dfree(c, w.AS); dfree(c, w.BS); dfree(c, w.G); dfree(c, w.gram_partials);
+  dfree(c, w.res); dfree(c, w.res_partials); dfree(c, w.counters); hfree(c, w.host);
+  const int64_t n = c->g.n_dofs;
+  VOX8_TRY(dalloc(c, &w.S, (size_t)3 * k * n));
+  VOX8_TRY(dalloc(c, &w.AS, (size_t)3 * k * n));
+  VOX8_TRY(dalloc(c, &w.BS, (size_t)3 * k * n));
+  VOX8_TRY(dalloc(c, &w.G, (size_t)(2 * 36 * 36 + 36 * 12 + 12 + 8)));
+  int64_t chunks = (n + 8191) / 8192;
+  if (chunks > 256) chunks = 256;
+  if (chunks < 1) chunks = 1;
+  w.gram_chunks = (int)chunks;
+  VOX8_TRY(dalloc(c, &w.gram_partials, (size_t)chunks * 2 * 36 * 36));
+  VOX8_TRY(dalloc(c, &w.res, 16));
+  VOX8_TRY(dalloc(c, &w.res_partials, (size_t)1024 * 12));
+  VOX8_TRY(dalloc(c, &w.counters, 4));
+  VOX8_HIP(hipMemset(w.counters, 0, sizeof(unsigned int) * 4));
+  VOX8_TRY(halloc(c, &w.host, 32));
+  w.kmax = k;
+  w.n = n;
+  return OK;
+}
+
+int sync_read(vox8_ctx* c, const double* d, double* h, int count) {
+  VOX8_HIP(hipMemcpyAsync(c->host, d, sizeof(double) * count, hipMemcpyDeviceToHost, c->st));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  for (int i = 0; i < count; ++i) h[i] = c->host[i];
+  return OK;
+}
+
+}  // namespace
+
+#define API_BEGIN try {
+#define API_END                                                   \
+  }                                                               \
+  catch (const std::bad_alloc&) {                                 \
+    set_error("host allocation failed");                          \
+    return ERR_HIP;                                               \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    set_error(std::string("exception: ") + e.what());             \
+    return ERR_STATE;                                             \
+  }
+
+extern "C" {
+
+const char* vox8_last_error(void) { return g_err.c_str(); }
+int vox8_version(void) { return 100; }
+
+int vox8_device_count(int* h_count) {
+  VOX8_CHECK(h_count != nullptr, "null output");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *h_count = (e == hipSuccess) ? n : 0;
+  return OK;
+}
+
+int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
+  API_BEGIN
+  VOX8_CHECK(out != nullptr, "null output");
+  VOX8_CHECK(nx > 0 && ny > 0 && nz > 0, "grid dimensions must be positive");
+  VOX8_CHECK((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "grid too large");
+  vox8_ctx* c = new vox8_ctx();
+  c->g = make_grid(nx, ny, nz);
+  c->st = (hipStream_t)hip_stream;
+  int s = 0;
+  auto fail = [&](int code) { vox8_destroy(c); *out = nullptr; return code; };
+  if (hipGetDevice(&c->device) != hipSuccess) { set_error("no HIP device"); return fail(ERR_HIP); }
+  const int64_t ne = c->g.n_elem, nn = c->g.n_nodes, nd = c->g.n_dofs;
+  if ((s = dalloc(c, &c->dKe, 576)) || (s = dalloc(c, &c->dMe, 576)) ||
+      (s = dalloc(c, &c->dKe32, 576)) || (s = dalloc(c, &c->dMe32, 576)) ||
+      (s = dalloc(c, &c->d_active, ne)) || (s = dalloc(c, &c->d_fixed, nn)) ||
+      (s = dalloc(c, &c->fK, ne)) || (s = dalloc(c, &c->fM, ne)) ||
+      (s = dalloc(c, &c->fK32, ne)) || (s = dalloc(c, &c->fM32, ne)) ||
+      (s = dalloc(c, &c->dinvK, nd)) || (s = dalloc(c, &c->dinvK32, nd)) ||
+      (s = dalloc(c, &c->red_partials, 4096 * 40)) || (s = dalloc(c, &c->red_counters, 64)) ||
+      (s = dalloc(c, &c->red_out, 64)) || (s = halloc(c, &c->host, 64)) ||
+      (s = dalloc(c, &c->d_oc, 8)) || (s = dalloc(c, &c->d_kphi, ne)) ||
+      (s = dalloc(c, &c->d_mphi, ne)) || (s = dalloc(c, &c->d_ce, ne)) ||
+      (s = dalloc(c, &c->d_sums, 32)) || (s = dalloc(c, &c->d_wts, 32)) ||
+      (s = dalloc(c, &c->d_den, ne)) || (s = dalloc(c, &c->d_fmask, ne)))
+    return fail(s);
+  if (hipMemset(c->red_counters, 0, 64 * sizeof(unsigned int)) != hipSuccess) {
+    set_error("hipMemset failed");
+    return fail(ERR_HIP);
+  }
+  *out = c;
+  return OK;
+  API_END
+}
+
+int vox8_destroy(vox8_ctx* c) {
+  if (!c) return OK;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st); else (void)hipDeviceSynchronize();
+  for (void* p : c->dev_allocs) (void)hipFree(p);
+  for (void* p : c->host_allocs) (void)hipHostFree(p);
+  delete c;
+  return OK;
+}
+
+int vox8_set_stream(vox8_ctx* c, void* hip_stream) {
+  VOX8_CHECK(c != nullptr, "null context");
+  c->st = (hipStream_t)hip_stream;
+  return OK;
+}
+
+int vox8_set_element_matrices(vox8_ctx* c, const double* h_Ke, const double* h_Me) {
+  API_BEGIN
+  VOX8_CHECK(c && h_Ke && h_Me, "null argument");
+  VOX8_HIP(hipSetDevice(c->device));
+  std::memcpy(c->Ke, h_Ke, sizeof(c->Ke));
+  std::memcpy(c->Me, h_Me, sizeof(c->Me));
+  float k32[576], m32[576];
+  for (int i = 0; i < 576; ++i) { k32[i] = (float)h_Ke[i]; m32[i] = (float)h_Me[i]; }
+  VOX8_HIP(hipMemcpyAsync(c->dKe, c->Ke, sizeof(c->Ke), hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipMemcpyAsync(c->dMe, c->Me, sizeof(c->Me), hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipMemcpyAsync(c->dKe32, k32, sizeof(k32), hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipMemcpyAsync(c->dMe32, m32, sizeof(m32), hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  c->have_mats = true;
+  c->dinv_valid = false;
+  return OK;
+  API_END
+}
+
+int vox8_set_topology(vox8_ctx* c, const uint8_t* h_active, const uint8_t* h_fixed) {
+  API_BEGIN
+  VOX8_CHECK(c && h_active && h_fixed, "null argument");
+  VOX8_HIP(hipSetDevice(c->device));
+  VOX8_HIP(hipMemcpyAsync(c->d_active, h_active, c->g.n_elem, hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipMemcpyAsync(c->d_fixed, h_fixed, c->g.n_nodes, hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  c->have_topo = true;
+  c->have_density = false;
+  c->dinv_valid = false;
+  return OK;
+  API_END
+}
+
+int vox8_set_tip_mass(vox8_ctx* c, const int64_t* h_nodes, int n_nodes, double mass_kg) {
+  API_BEGIN
+  VOX8_CHECK(c != nullptr, "null context");
+  VOX8_CHECK(n_nodes >= 0, "negative node count");
+  VOX8_HIP(hipSetDevice(c->device));
+  if (n_nodes == 0 || mass_kg <= 0.0) { c->n_tip = 0; c->tip_m = 0.0; return OK; }
+  VOX8_CHECK(h_nodes != nullptr, "null node list");
+  for (int i = 0; i < n_nodes; ++i)
+    VOX8_CHECK(h_nodes[i] >= 0 && h_nodes[i] < c->g.n_nodes, "tip node out of range");
+  if (n_nodes > c->tip_cap) {
+    dfree(c, c->d_tip);
+    VOX8_TRY(dalloc(c, &c->d_tip, n_nodes));
+    c->tip_cap = n_nodes;
+  }
+  VOX8_HIP(hipMemcpyAsync(c->d_tip, h_nodes, sizeof(int64_t) * n_nodes, hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  c->n_tip = n_nodes;
+  c->tip_m = mass_kg / n_nodes;
+  return OK;
+  API_END
+}
+
+int vox8_set_density(vox8_ctx* c, const double* d_rho, double p, double e_min, double mthr,
+                     double mq) {
+  API_BEGIN
+  VOX8_TRY(require(c, false));
+  VOX8_CHECK(d_rho != nullptr, "null density");
+  VOX8_TRY(launch_element_factors(c->g, d_rho, c->d_active, p, e_min, mthr, mq, c->fK, c->fM,
+                                  c->fK32, c->fM32, c->st));
+  c->have_density = true;
+  c->dinv_valid = false;
+  return OK;
+  API_END
+}
+
+int vox8_apply(vox8_ctx* c, int op, int prec, const void* d_X, void* d_YK, void* d_YM, int nrhs,
+               int64_t ld) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(op >= 1 && op <= 3, "op must be K(1), M(2) or KM(3)");
+  VOX8_CHECK(nrhs >= 1 && nrhs <= 65535, "nrhs out of range");
+  VOX8_CHECK(ld >= c->g.n_dofs, "ld < n_dofs");
+  VOX8_CHECK(d_X && (!(op & OP_K) || d_YK) && (!(op & OP_M) || d_YM), "null buffer");
+  if (prec == VOX8_PREC_FP32)
+    return apply_op<float>(c, op, (const float*)d_X, ld, (float*)d_YK, (float*)d_YM, ld, nrhs,
+                           DotFuse());
+  VOX8_CHECK(prec == VOX8_PREC_FP64, "prec must be FP64 (0) or FP32 (2) for vox8_apply");
+  return apply_op<double>(c, op, (const double*)d_X, ld, (double*)d_YK, (double*)d_YM, ld, nrhs,
+                          DotFuse());
+  API_END
+}
+
+int vox8_diagonal(vox8_ctx* c, int op, double* d_diag) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(op == 1 || op == 2, "op must be K(1) or M(2)");
+  VOX8_CHECK(d_diag != nullptr, "null output");
+  if (op == 1)
+    return launch_jacobi(c->g, c->fK, c->dKe, c->d_fixed, 0, nullptr, 0, 0.0, d_diag, nullptr, c->st);
+  return launch_jacobi(c->g, c->fM, c->dMe, c->d_fixed, 0, c->d_tip, c->n_tip, c->tip_m, d_diag,
+                       nullptr, c->st);
+  API_END
+}
+
+int vox8_pcg(vox8_ctx* c, const double* d_F, double* d_U, int nrhs, int64_t ld, int x0_given,
+             double rtol, int maxiter, int prec, int* h_iters, int* h_info, double* h_relres) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_F && d_U, "null buffer");
+  VOX8_CHECK(nrhs >= 1 && nrhs <= 64, "nrhs must be 1..64");
+  VOX8_CHECK(ld == c->g.n_dofs, "pcg: ld must equal n_dofs");
+  VOX8_CHECK(rtol > 0.0 && maxiter >= 1, "rtol must be > 0 and maxiter >= 1");
+  VOX8_TRY(ensure_dinv(c, 0));
+  VOX8_TRY(ensure_pcg_work(c, c->w64, nrhs));
+  std::vector<PcgResult> res(nrhs);
+  PcgOps<double> A64;
+  A64.g = &c->g;
+  A64.fixed = c->d_fixed;
+  A64.dinv = c->dinvK;
+  A64.apply = [c](const double* x, int64_t ldx, double* y, int64_t ldy, int nr, hipStream_t,
+                  DotFuse f) { return apply_op<double>(c, OP_K, x, ldx, y, nullptr, ldy, nr, f); };
+  if (prec == VOX8_PREC_MIXED) {
+    VOX8_TRY(ensure_pcg_work(c, c->w32, nrhs));
+    PcgOps<float> A32;
+    A32.g = &c->g;
+    A32.fixed = c->d_fixed;
+    A32.dinv = c->dinvK32;
+    A32.apply = [c](const float* x, int64_t ldx, float* y, int64_t ldy, int nr, hipStream_t,
+                    DotFuse f) { return apply_op<float>(c, OP_K, x, ldx, y, nullptr, ldy, nr, f); };
+    VOX8_TRY(pcg_mixed(A64, A32, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, 1e-4, c->w64,
+                       c->w32, c->st, res.data()));
+  } else {
+    VOX8_CHECK(prec == VOX8_PREC_FP64, "prec must be FP64 (0) or MIXED (1)");
+    VOX8_TRY(pcg_fp64(A64, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, c->w64, c->st,
+                      res.data()));
+  }
+  for (int j = 0; j < nrhs; ++j) {
+    if (h_iters) h_iters[j] = res[j].iters;
+    if (h_info) h_info[j] = res[j].info;
+    if (h_relres) h_relres[j] = res[j].relres;
+  }
+  return OK;
+  API_END
+}
+
+int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const double* d_X0,
+                double* d_X, double* h_lam, int* h_iters, int* h_info, double* h_max_res,
+                uint64_t seed) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_X && h_lam, "null buffer");
+  VOX8_CHECK(k >= 1 && k <= 12, "k must be 1..12");
+  VOX8_CHECK(tol > 0.0 && maxiter >= 0, "tol must be > 0");
+  VOX8_CHECK(prec == VOX8_PREC_FP64 || prec == VOX8_PREC_MIXED, "prec must be FP64 or MIXED");
+  VOX8_TRY(ensure_dinv(c, 1));
+  VOX8_TRY(ensure_lobpcg_work(c, k));
+  PcgOps<double> A;
+  A.g = &c->g;
+  A.fixed = c->d_fixed;
+  A.dinv = c->dinvK;
+  auto km = [c](const double* x, double* yK, double* yM, int nr, hipStream_t) {
+    return apply_op<double>(c, OP_KM, x, c->g.n_dofs, yK, yM, c->g.n_dofs, nr, DotFuse());
+  };
+  LobpcgResult r{};
+  VOX8_TRY(lobpcg_fp64(A, km, k, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed));
+  if (h_iters) *h_iters = r.iters;
+  if (h_info) *h_info = r.info;
+  if (h_max_res) *h_max_res = r.max_res;
+  return OK;
+  API_END
+}
+
+int vox8_sensitivities(vox8_ctx* c, const double* d_U, int nrhs, int64_t ld, const double* h_w,
+                       const double* d_phi, double lam, const double* d_rho, double p, double e_min,
+                       double mthr, double mq, double* d_ce, double* d_dc, double* d_dlam,
+                       double* h_comp, double* h_vmv) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(nrhs >= 0 && nrhs <= 16, "nrhs must be 0..16");
+  VOX8_CHECK(nrhs == 0 || (d_U && h_w), "null U or weights");
+  VOX8_CHECK(ld >= c->g.n_dofs || nrhs == 0, "ld < n_dofs");
+  VOX8_CHECK((!d_dc && !d_dlam) || d_rho, "derivatives need d_rho");
+  double w[16] = {0};
+  for (int j = 0; j < nrhs; ++j) w[j] = h_w[j];
+  VOX8_HIP(hipMemcpyAsync(c->d_wts, w, sizeof(w), hipMemcpyHostToDevice, c->st));
+  double* ce = d_ce ? d_ce : c->d_ce;
+  VOX8_TRY(launch_quadforms(c->g, c->fK, c->fM, c->d_active, c->dKe, c->dMe, d_U, ld, nrhs,
+                            c->d_wts, d_phi, nrhs > 0 ? ce : nullptr, c->d_kphi, c->d_mphi,
+                            c->red_partials, c->red_counters, c->d_sums, c->d_tip, c->n_tip,
+                            c->tip_m, c->st));
+  if (d_dc || d_dlam)
+    VOX8_TRY(launch_sens_finish(c->g.n_elem, c->d_active, d_rho, p, e_min, mthr, mq, ce,
+                                nrhs > 0 ? d_dc : nullptr, c->d_kphi, c->d_mphi, lam,
+                                c->d_sums + nrhs, d_phi ? d_dlam : nullptr, c->st));
+  double h[17];
+  VOX8_TRY(sync_read(c, c->d_sums, h, nrhs + 1));
+  if (h_comp) for (int j = 0; j < nrhs; ++j) h_comp[j] = h[j];
+  if (h_vmv) *h_vmv = d_phi ? h[nrhs] : 0.0;
+  return OK;
+  API_END
+}
+
+int vox8_filter_setup(vox8_ctx* c, double r_min_vox, const uint8_t* d_mask) {
+  API_BEGIN
+  VOX8_CHECK(c && d_mask, "null argument");
+  VOX8_CHECK(r_min_vox > 0.0 && r_min_vox < 32.0, "filter radius must be in (0, 32) voxels");
+  VOX8_HIP(hipSetDevice(c->device));
+  const int r = (int)std::ceil(r_min_vox);
+  std::vector<int> off;
+  std::vector<double> w;
+  for (int dz = -r; dz <= r; ++dz)
+    for (int dy = -r; dy <= r; ++dy)
+      for (int dx = -r; dx <= r; ++dx) {
+        const double wt = r_min_vox - std::sqrt((double)(dx * dx + dy * dy + dz * dz));
+        if (wt > 0.0) { off.push_back(dx); off.push_back(dy); off.push_back(dz); w.push_back(wt); }
+      }
+  dfree(c, c->d_toff);
+  dfree(c, c->d_tw);
+  c->d_toff = nullptr;
+  c->d_tw = nullptr;
+  c->ntap = (int)w.size();
+  VOX8_TRY(dalloc(c, &c->d_toff, off.size()));
+  VOX8_TRY(dalloc(c, &c->d_tw, w.size()));
+  VOX8_HIP(hipMemcpyAsync(c->d_toff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipMemcpyAsync(c->d_tw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, c->st));
+  VOX8_HIP(hipMemcpyAsync(c->d_fmask, d_mask, c->g.n_elem, hipMemcpyDeviceToDevice, c->st));
+  VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask,
+                         nullptr, nullptr, c->d_den, c->st));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  c->have_filter = true;
+  return OK;
+  API_END
+}
+
+int vox8_filter(vox8_ctx* c, const double* d_in, double* d_out) {
+  API_BEGIN
+  VOX8_CHECK(c && d_in && d_out, "null argument");
+  VOX8_CHECK(d_in != d_out, "filter cannot run in place");
+  if (!c->have_filter) { set_error("filter not set up (vox8_filter_setup)"); return ERR_STATE; }
+  return launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask, c->d_den,
+                       d_in, d_out, c->st);
+  API_END
+}
+
+int vox8_oc_update(vox8_ctx* c, const uint8_t* d_design, const double* d_x, const double* d_dc,
+                   double V_e, double target, double move, double* d_xnew, double* h_lmid) {
+  API_BEGIN
+  VOX8_CHECK(c && d_design && d_x && d_dc && d_xnew, "null argument");
+  VOX8_CHECK(V_e > 0.0, "V_e must be positive");
+  VOX8_TRY(launch_oc(c->g.n_elem, d_design, d_x, d_dc, V_e, target, move, c->d_oc, c->red_partials,
+                     c->red_counters, d_xnew, 100, c->st));
+  if (h_lmid) {
+    double h[5];
+    VOX8_TRY(sync_read(c, c->d_oc, h, 5));
+    *h_lmid = h[2];
+  }
+  return OK;
+  API_END
+}
+
+int vox8_stats(vox8_ctx* c, int64_t n, const uint8_t* d_mask, const double* d_a, const double* d_b,
+               double* h_out4) {
+  API_BEGIN
+  VOX8_CHECK(c && d_a && h_out4 && n > 0, "bad argument");
+  VOX8_TRY(launch_stats(n, d_mask, d_a, d_b, c->red_partials, c->red_counters, c->red_out, c->st));
+  return sync_read(c, c->red_out, h_out4, 4);
+  API_END
+}
+
+int vox8_block_dots(vox8_ctx* c, int64_t n, int k, const double* d_X, int64_t ld, const double* d_v,
+                    double* h_out) {
+  API_BEGIN
+  VOX8_CHECK(c && d_X && d_v && h_out && n > 0 && ld >= n, "bad argument");
+  VOX8_TRY(launch_block_dots(n, k, d_X, ld, d_v, c->red_partials, c->red_counters, c->red_out, c->st));
+  return sync_read(c, c->red_out, h_out, 2 * k + 1);
+  API_END
+}
+
+int vox8_axpby(vox8_ctx* c, int64_t n, const uint8_t* d_mask, double a, const double* d_x, double b,
+               double* d_y) {
+  API_BEGIN
+  VOX8_CHECK(c && d_x && d_y && n > 0, "bad argument");
+  return launch_axpby(n, d_mask, a, d_x, b, d_y, c->st);
+  API_END
+}
+
+int vox8_compose_density(vox8_ctx* c, const uint8_t* d_design, const uint8_t* d_in_arm,
+                         const double* d_filt, double* d_out) {
+  API_BEGIN
+  VOX8_CHECK(c && d_design && d_in_arm && d_filt && d_out, "null argument");
+  return launch_compose(c->g.n_elem, d_design, d_in_arm, d_filt, d_out, c->st);
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// vox8 — solver-side interfaces (batched PCG, LOBPCG) used by the C-ABI layer.
+#pragma once
+#include <functional>
+
+#include "vox8_kernels.h"
+
+namespace vox8 {
+
+// Operator bundle a solver runs on: y = A x for `nrhs` column-major vectors.
+template <typename T>
+struct PcgOps {
+  const Grid* g = nullptr;
+  const uint8_t* fixed = nullptr;   // per-node fixed-DOF bitmask
+  const T* dinv = nullptr;          // Jacobi inverse diagonal (0 at fixed DOFs)
+  std::function<int(const T* x, int64_t ldx, T* y, int64_t ldy, int nrhs, hipStream_t st,
+                    DotFuse fuse)>
+      apply;
+};
+
+template <typename T>
+struct PcgWork {
+  T* r = nullptr;
+  T* p = nullptr;
+  T* q = nullptr;
+  T* x = nullptr;                 // only used as the correction of the mixed solve
+  double* rhs64 = nullptr;        // FP64 right-hand side of the inner solve (+nrhs scales)
+  PcgScalars* scal = nullptr;     // device [ncols]
+  PcgScalars* host_scal = nullptr;  // pinned [ncols]
+  double* host_scale = nullptr;     // pinned [ncols]
+  double* partials = nullptr;
+  unsigned int* counters = nullptr;
+  int ncols = 0;
+  int64_t n = 0;
+};
+
+struct PcgResult {
+  int iters;      // CG iterations (summed over inner solves for the mixed path)
+  int info;       // 0 converged, >0 not converged (scipy convention)
+  double relres;  // final ||r|| / ||b|| (FP64 residual)
+  int outer;      // refinement steps (mixed path), 1 for the plain path
+};
+
+int pcg_fp64(const PcgOps<double>& A, const double* F, int64_t ldf, double* X, bool x0_given,
+             int nrhs, double rtol, int maxiter, PcgWork<double>& w, hipStream_t st,
+             PcgResult* res);
+int pcg_mixed(const PcgOps<double>& A64, const PcgOps<float>& A32, const double* F, int64_t ldf,
+              double* X, bool x0_given, int nrhs, double rtol, int maxiter, double inner_rtol,
+              PcgWork<double>& w64, PcgWork<float>& w32, hipStream_t st, PcgResult* res);
+int scale_columns_to_float(const double* a, int64_t lda, double* b, const double* host_scale,
+                           int nrhs, int64_t n, hipStream_t st);
+int convert_to_float(const double* a, float* b, int64_t n, hipStream_t st);
+
+// ---- LOBPCG ----
+struct LobpcgResult {
+  int iters;
+  int info;            // 0 converged, >0 not converged (iterations done), <0 failure
+  double max_res;      // final max absolute residual ||K x - lam M x||
+};
+
+struct LobpcgWork {
+  double* S = nullptr;    // [3k][n]  X | W | P
+  double* AS = nullptr;   // [3k][n]
+  double* BS = nullptr;   // [3k][n]
+  double* G = nullptr;    // device: GA (m*m) | GB (m*m) | C (m*k) | lam (k) | status
+  double* gram_partials = nullptr;
+  double* res = nullptr;      // device residual norms^2 [k]
+  double* host = nullptr;     // pinned mirror of lam|res|status
+  double* res_partials = nullptr;
+  unsigned int* counters = nullptr;
+  int kmax = 0;
+  int64_t n = 0;
+  int gram_chunks = 0;
+};
+
+int lobpcg_fp64(const PcgOps<double>& A, const std::function<int(const double*, double*, double*,
+                                                                 int, hipStream_t)>& applyKM,
+                int k, double tol, int maxiter, const double* X0, double* Xout, double* lam_out,
+                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed);
+
+}  // namespace vox8

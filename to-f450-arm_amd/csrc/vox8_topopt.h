@@ -1,0 +1,29 @@
+// vox8 — launchers of the SIMP-loop kernels (vox8_topopt.hip).
+#pragma once
+#include "vox8_kernels.h"
+
+namespace vox8 {
+int launch_filter(int nx, int ny, int nz, int ntap, const int* toff, const double* tw,
+                  const uint8_t* mask, const double* den, const double* in, double* out,
+                  hipStream_t st);
+int launch_quadforms(const Grid& g, const double* fK, const double* fM, const uint8_t* active,
+                     const double* Ke, const double* Me, const double* U, int64_t ldu, int nrhs,
+                     const double* wts, const double* phi, double* ce, double* kphi,
+                     double* mphi, double* partials, unsigned int* counter, double* sums,
+                     const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st);
+int launch_sens_finish(int64_t n, const uint8_t* active, const double* rho, double p, double emin,
+                       double mthr, double mq, const double* ce, double* dc, const double* kphi,
+                       const double* mphi, double lam, const double* vmv, double* dlam,
+                       hipStream_t st);
+int launch_oc(int64_t n, const uint8_t* design, const double* x, const double* dc, double Ve,
+              double target, double move, double* state, double* partials, unsigned int* counter,
+              double* xnew, int max_steps, hipStream_t st);
+int launch_stats(int64_t n, const uint8_t* mask, const double* a, const double* b, double* partials,
+                 unsigned int* counter, double* out, hipStream_t st);
+int launch_block_dots(int64_t n, int k, const double* X, int64_t ldx, const double* v,
+                      double* partials, unsigned int* counter, double* out, hipStream_t st);
+int launch_axpby(int64_t n, const uint8_t* mask, double a, const double* x, double b, double* y,
+                 hipStream_t st);
+int launch_compose(int64_t n, const uint8_t* design, const uint8_t* in_arm, const double* filt,
+                   double* out, hipStream_t st);
+}  // namespace vox8

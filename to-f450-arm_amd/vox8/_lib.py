@@ -1,0 +1,92 @@
+"""ctypes binding of libvox8.so (C ABI in include/vox8.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if it is missing or no GPU is visible, every GPU entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libvox8.so"
+
+OP_K, OP_M, OP_KM = 1, 2, 3
+PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
+
+_vp = C.c_void_p
+_i32 = C.c_int
+_i64 = C.c_int64
+_f64 = C.c_double
+_pi32 = C.POINTER(C.c_int)
+_pf64 = C.POINTER(C.c_double)
+
+# name -> argtypes (all return int status, except vox8_last_error)
+SIGNATURES = {
+    "vox8_version": [],
+    "vox8_device_count": [_pi32],
+    "vox8_create": [C.POINTER(_vp), _i32, _i32, _i32, _vp],
+    "vox8_destroy": [_vp],
+    "vox8_set_stream": [_vp, _vp],
+    "vox8_set_element_matrices": [_vp, _pf64, _pf64],
+    "vox8_set_topology": [_vp, _vp, _vp],
+    "vox8_set_tip_mass": [_vp, C.POINTER(C.c_int64), _i32, _f64],
+    "vox8_set_density": [_vp, _vp, _f64, _f64, _f64, _f64],
+    "vox8_apply": [_vp, _i32, _i32, _vp, _vp, _vp, _i32, _i64],
+    "vox8_diagonal": [_vp, _i32, _vp],
+    "vox8_pcg": [_vp, _vp, _vp, _i32, _i64, _i32, _f64, _i32, _i32, _pi32, _pi32, _pf64],
+    "vox8_lobpcg": [_vp, _i32, _f64, _i32, _i32, _vp, _vp, _pf64, _pi32, _pi32, _pf64,
+                    C.c_uint64],
+    "vox8_sensitivities": [_vp, _vp, _i32, _i64, _pf64, _vp, _f64, _vp, _f64, _f64, _f64,
+                           _f64, _vp, _vp, _vp, _pf64, _pf64],
+    "vox8_filter_setup": [_vp, _f64, _vp],
+    "vox8_filter": [_vp, _vp, _vp],
+    "vox8_oc_update": [_vp, _vp, _vp, _vp, _f64, _f64, _f64, _vp, _pf64],
+    "vox8_stats": [_vp, _i64, _vp, _vp, _vp, _pf64],
+    "vox8_block_dots": [_vp, _i64, _i32, _vp, _i64, _vp, _pf64],
+    "vox8_axpby": [_vp, _i64, _vp, _f64, _vp, _f64, _vp],
+    "vox8_compose_density": [_vp, _vp, _vp, _vp, _vp],
+}
+
+
+class Vox8Error(RuntimeError):
+    """A vox8 C-ABI call returned an error status."""
+
+
+_lib = None
+
+
+def load():
+    """Load libvox8.so once and attach the ctypes signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise Vox8Error(f"{LIB_PATH} is not built; run vox8.build.build() (or __graft_entry__.build())")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, argtypes in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = C.c_int
+    lib.vox8_last_error.argtypes = []
+    lib.vox8_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call a vox8 entry point and raise Vox8Error on a non-zero status."""
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st != 0:
+        msg = lib.vox8_last_error().decode(errors="replace")
+        raise Vox8Error(f"{name} failed (status {st}): {msg}")
+    return st
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def f64p(arr):
+    return arr.ctypes.data_as(_pf64)
