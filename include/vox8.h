@@ -51,6 +51,12 @@ int vox8_set_stream(vox8_ctx* ctx, void* hip_stream);
  * has the box-element block structure, else the general 24x24 stencil. */
 int vox8_set_element_matrices(vox8_ctx* ctx, const double* h_Ke, const double* h_Me);
 
+/* Stencil selection: kind 0 = automatic (structured Hadamard-basis kernel
+ * when Ke/Me have the box-element structure, else dense), 1 = always the
+ * general dense 24x24 kernel.  *h_pattern (may be NULL) receives the kernel
+ * in use: -1 dense, 0 orthotropic pattern (45 non-zeros), 1 general blocks. */
+int vox8_set_stencil(vox8_ctx* ctx, int kind, int* h_pattern);
+
 /* Active elements (h_elem_active[n_elem], 1 = in the model) and the fixed-DOF
  * bitmask per node (bit c = DOF 3n+c fixed; ghost nodes must be fixed).
  * Replaces the masks / fixed_dofs / ghost-DOF logic of src/fem.py:225-305. */
@@ -95,6 +101,18 @@ int vox8_pcg(vox8_ctx* ctx, const double* d_F, double* d_U, int nrhs, int64_t ld
 int vox8_lobpcg(vox8_ctx* ctx, int k, double tol, int maxiter, int prec, const double* d_X0,
                 double* d_X, double* h_lam, int* h_iters, int* h_info, double* h_max_res,
                 uint64_t seed);
+
+/* The geometric-multigrid preconditioner used by the MIXED solvers:
+ * z = V-cycle(r) for nrhs FP64 column-major vectors (ld = n_dofs); r and z
+ * may alias.  No reference counterpart: the reference preconditions with the
+ * Jacobi diagonal (src/fem.py:441-443, 543-547). */
+int vox8_precond_apply(vox8_ctx* ctx, const double* d_r, double* d_z, int nrhs);
+/* Level hierarchy: element counts per level (h_dims[3*l..3*l+2]) and the
+ * smoother's lambda_max(D^-1 K) estimate per level. */
+int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, int max_levels);
+/* Multigrid options: Chebyshev steps before/after (nu), smoothing interval
+ * [lam_max/alpha, lam_max], coarsest-level size (DOFs) for the dense inverse. */
+int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs);
 
 /* Adjoint quantities for the current density:
  *   d_ce[e]  = sum_j w_j U_j,e^T Ke U_j,e      (elem_compliance, src/fem.py:605-615)

@@ -1,0 +1,147 @@
+"""NumPy restatement of the vox8 multigrid V-cycle (test infrastructure).
+
+Same hierarchy and operators as to-f450-arm_amd/csrc/vox8_mg.hip, written
+with assembled sparse matrices so the preconditioner can be checked on the
+CPU (symmetry, positivity, contraction) and compared with the device kernel:
+levels coarsened 2:1 (ceil), factor = mean of 8 children, element matrix
+2^l Ke, trilinear P (masked by fixed DOFs on both sides), R = P^T, Chebyshev
+smoothing of D^-1 K on [lam_max/alpha, lam_max], dense coarsest solve.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+
+LOCAL = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+
+
+def assemble(nx, ny, nz, f, Ke, fixed):
+    """Free-DOF-embedded K (identity-free: rows/cols of fixed DOFs zero)."""
+    NX, NY = nx + 1, ny + 1
+    ez, ey, ex = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    nodes = (ex[:, None] + LOCAL[None, :, 0]) + NX * ((ey[:, None] + LOCAL[None, :, 1]) + NY * (ez[:, None] + LOCAL[None, :, 2]))
+    dofs = (3 * nodes[:, :, None] + np.arange(3)).reshape(-1, 24)
+    rows = np.repeat(dofs, 24, axis=1).ravel()
+    cols = np.tile(dofs, (1, 24)).ravel()
+    vals = (f[:, None] * Ke.ravel()[None, :]).ravel()
+    n = 3 * NX * NY * (nz + 1)
+    K = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    free = ~np.repeat(fixed, 3).astype(bool) if fixed.dtype == bool else np.array(
+        [not ((fixed[i // 3] >> (i % 3)) & 1) for i in range(n)])
+    D = sp.diags(free.astype(float))
+    return (D @ K @ D).tocsr(), free
+
+
+def prolongation(g_f, g_c, free_f, free_c):
+    (nxf, nyf, nzf), (nxc, nyc, nzc) = g_f, g_c
+
+    def axis(nf, nc):
+        P = np.zeros((nf + 1, nc + 1))
+        for i in range(nf + 1):
+            if i % 2 == 0:
+                P[i, i // 2] = 1.0
+            else:
+                P[i, (i - 1) // 2] = 0.5
+                P[i, (i + 1) // 2] = 0.5
+        return sp.csr_matrix(P)
+    Pn = sp.kron(axis(nzf, nzc), sp.kron(axis(nyf, nyc), axis(nxf, nxc)))
+    P = sp.kron(Pn, sp.eye(3)).tocsr()
+    return (sp.diags(free_f.astype(float)) @ P @ sp.diags(free_c.astype(float))).tocsr()
+
+
+class VCycle:
+    def __init__(self, dims, f, Ke, fixed_nodes_mask, active=None, nu=2, alpha=12.0, coarse_dofs=700):
+        self.nu, self.alpha = nu, alpha
+        nx, ny, nz = dims
+        fixed = fixed_nodes_mask.astype(np.uint8) * 7 if fixed_nodes_mask.dtype == bool else fixed_nodes_mask
+        act = np.ones(nx * ny * nz, bool) if active is None else active.astype(bool)
+        self.levels = []
+        scale = 1.0
+        while True:
+            K, free = assemble(nx, ny, nz, f, scale * Ke, fixed)
+            d = K.diagonal()
+            dinv = np.where(free, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+            self.levels.append(dict(dims=(nx, ny, nz), K=K, free=free, dinv=dinv))
+            n = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+            if n <= coarse_dofs or min(nx, ny, nz) < 2:
+                break
+            cx, cy, cz = (nx + 1) // 2, (ny + 1) // 2, (nz + 1) // 2
+            fc = np.zeros(cx * cy * cz)
+            ac = np.zeros(cx * cy * cz, bool)
+            for Z in range(cz):
+                for Y in range(cy):
+                    for X in range(cx):
+                        s, a = 0.0, False
+                        for k in range(8):
+                            x, y, z = 2 * X + (k & 1), 2 * Y + ((k >> 1) & 1), 2 * Z + (k >> 2)
+                            if x < nx and y < ny and z < nz:
+                                e = x + nx * (y + ny * z)
+                                s += f[e]
+                                a |= act[e]
+                        fc[X + cx * (Y + cy * Z)] = s / 8
+                        ac[X + cx * (Y + cy * Z)] = a
+            used = np.zeros((cz + 1, cy + 1, cx + 1), bool)
+            a3 = ac.reshape(cz, cy, cx)
+            for oz in (0, 1):
+                for oy in (0, 1):
+                    for ox in (0, 1):
+                        used[oz:oz + cz, oy:oy + cy, ox:ox + cx] |= a3
+            fx3 = fixed.reshape(nz + 1, ny + 1, nx + 1)
+            fc_fix = np.zeros((cz + 1, cy + 1, cx + 1), np.uint8)
+            for K_ in range(cz + 1):
+                for J in range(cy + 1):
+                    for I in range(cx + 1):
+                        if 2 * I <= nx and 2 * J <= ny and 2 * K_ <= nz:
+                            fc_fix[K_, J, I] = fx3[2 * K_, 2 * J, 2 * I]
+            fc_fix[~used] = 7
+            fixed_c = fc_fix.ravel()
+            _, free_c = assemble(cx, cy, cz, fc, Ke, fixed_c)
+            self.levels[-1]["P"] = prolongation((nx, ny, nz), (cx, cy, cz), free, free_c)
+            nx, ny, nz, f, fixed, act = cx, cy, cz, fc, fixed_c, ac
+            scale *= 2.0
+        for L in self.levels[:-1]:
+            A = sp.diags(L["dinv"]) @ L["K"]
+            L["lam_max"] = 1.1 * float(np.max(np.abs(np.linalg.eigvals(A.toarray())))) if A.shape[0] < 3000 \
+                else 1.1 * _power(A)
+        C = self.levels[-1]
+        Kc = C["K"].toarray()
+        Kc[~C["free"], :] = 0
+        Kc[:, ~C["free"]] = 0
+        Kc[~C["free"], ~C["free"]] = 1.0
+        C["inv"] = np.linalg.inv(Kc)
+
+    def smooth(self, L, x, r, d, nu):
+        lmax = L["lam_max"]
+        lmin = lmax / self.alpha
+        theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+        sigma = theta / delta
+        rho = 1.0 / sigma
+        for _ in range(nu):
+            rho1 = 1.0 / (2 * sigma - rho)
+            x = x + d
+            r = r - L["K"] @ d
+            d = rho1 * rho * d + 2 * rho1 / delta * L["dinv"] * r
+            rho = rho1
+        return x, r
+
+    def __call__(self, b, l=0):
+        L = self.levels[l]
+        if l == len(self.levels) - 1:
+            return np.where(L["free"], L["inv"] @ b, 0.0)
+        theta = 0.5 * (L["lam_max"] + L["lam_max"] / self.alpha)
+        x, r = self.smooth(L, np.zeros_like(b), b.copy(), L["dinv"] * b / theta, self.nu)
+        x = x + L["P"] @ self(L["P"].T @ r, l + 1)
+        r = b - L["K"] @ x
+        x, _ = self.smooth(L, x, r, L["dinv"] * r / theta, self.nu)
+        return x
+
+
+def _power(A, it=60):
+    v = np.random.default_rng(0).standard_normal(A.shape[0])
+    lam = 0.0
+    for _ in range(it):
+        w = A @ v
+        lam = np.linalg.norm(w)
+        v = w / lam
+    return lam

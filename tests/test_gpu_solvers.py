@@ -79,7 +79,8 @@ def test_pcg_multi_rhs_and_warm_start(gpu):
     assert max(info2["iters"]) <= 2
     # zero right-hand side returns zeros immediately (scipy: bnrm2 == 0)
     Z, infoz = vfem.solve_block(rho, np.zeros((1, geom.n_dofs)), x0=U[:1], rtol=1e-8)
-    assert infoz["info"] == [0] and float(Z.abs().max()) == 0.0
+    zmax = float(Z.abs().max())
+    assert infoz["info"] == [0] and zmax == 0.0, zmax
 
 
 def test_pcg_maxiter_reports_info(gpu):
@@ -101,7 +102,11 @@ def test_lobpcg_matches_dense_eigh(gpu, k):
     Kff = K[free][:, free].toarray()
     Mff = M[free][:, free].toarray()
     lam_ref, V = sla.eigh(Kff, Mff, subset_by_index=[0, k - 1])
-    f, modes, lam = vfem.modal_analysis(rho, n_modes=k, tip_mass_kg=0.0, tol=1e-7, maxiter=2000)
+    # absolute residual tol (scipy semantics): 1e-9 of ||K|| ~ E h, well above the
+    # FP64 rounding floor of K x for this size
+    tol = 1e-9 * 1.7e9 * geom.dx
+    f, modes, lam = vfem.modal_analysis(rho, n_modes=k, tip_mass_kg=0.0, tol=tol, maxiter=3000)
+    assert vfem.last_info["lobpcg"]["info"] == 0
     assert np.max(np.abs(lam / lam_ref - 1)) < 1e-6
     for j in range(k):
         assert mac(modes[free, j], V[:, j]) > 0.9999
@@ -120,8 +125,11 @@ def test_lobpcg_tip_mass_and_warm_start(gpu):
     free = ofem.free_dofs
     lam_ref, V = sla.eigh(K[free][:, free].toarray(), M[free][:, free].toarray(),
                           subset_by_index=[0, 3])
-    f, modes, lam = vfem.modal_analysis(rho, 4, 1e-6, tip, tol=1e-7, maxiter=2000)
+    tol = 1e-8 * 1.7e9 * geom.dx
+    f, modes, lam = vfem.modal_analysis(rho, 4, 1e-6, tip, tol=tol, maxiter=6000)
+    assert vfem.last_info["lobpcg"]["info"] == 0
     assert np.max(np.abs(lam / lam_ref - 1)) < 1e-6
-    f2, modes2, lam2 = vfem.modal_analysis(rho, 4, 1e-6, tip, X0_free=modes[free], tol=1e-7)
-    assert vfem.last_info["lobpcg"]["iters"] <= 3
+    first = vfem.last_info["lobpcg"]["iters"]
+    f2, modes2, lam2 = vfem.modal_analysis(rho, 4, 1e-6, tip, X0_free=modes[free], tol=tol)
+    assert vfem.last_info["lobpcg"]["iters"] <= max(3, first // 10)
     assert np.max(np.abs(lam2 / lam_ref - 1)) < 1e-6

@@ -97,3 +97,40 @@ def test_mutated_Ke_is_used(gpu):
     ref = np.zeros_like(x)
     ref[ofem.free_dofs] = (K @ x)[ofem.free_dofs]
     assert rel(vfem.assemble_K(rho) @ x, ref) < 1e-13
+
+
+def _general_D():
+    """A full (triclinic) elasticity matrix: SPD with shear-normal coupling."""
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((6, 6))
+    return (A @ A.T + 6 * np.eye(6)) * 1e8
+
+
+@pytest.mark.parametrize("kind", ["iso", "ortho", "general"])
+def test_fast_stencil_patterns_vs_dense(gpu, kind):
+    """The structured kernel (each pattern) against the dense 24x24 kernel and the oracle."""
+    from oracle.element import element_K, ortho_D
+    geom = MaskedBox(40, 11, 9, seed=3)            # spans several tiles in x, y and z
+    ofem, vfem, rho = _pair(geom)
+    if kind != "iso":
+        D = ortho_D(2.0e9, 1.2e9, 1.0e9, 0.35, 0.3, 0.33, 0.6e9, 0.4e9, 0.5e9) if kind == "ortho" else _general_D()
+        Ke = element_K(0, 0, geom.dx, geom.dy, geom.dz, D=D)
+        ofem.Ke_solid, ofem.Ke_flat = Ke, Ke.ravel()
+        vfem.Ke_solid = Ke.copy()
+    pat = vfem.set_stencil("auto")
+    assert pat == {"iso": 0, "ortho": 0, "general": 1}[kind]
+    X = np.random.default_rng(7).standard_normal((geom.n_dofs, 3))
+    yk = vfem.assemble_K(rho) @ X
+    ym = vfem.assemble_M(rho) @ X
+    assert vfem.set_stencil("dense") == -1
+    yk_d = vfem.assemble_K(rho) @ X
+    ym_d = vfem.assemble_M(rho) @ X
+    vfem.set_stencil("auto")
+    assert rel(yk, yk_d) < 1e-13 and rel(ym, ym_d) < 1e-13
+    K, M = ofem.assemble_K(rho), ofem.assemble_M(rho)
+    free = ofem.free_dofs
+    Xf = np.zeros_like(X)
+    Xf[free] = X[free]
+    ref = np.zeros_like(X)
+    ref[free] = (K @ Xf)[free]
+    assert rel(yk, ref) < 1e-13

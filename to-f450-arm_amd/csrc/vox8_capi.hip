@@ -4,7 +4,9 @@
 #include <new>
 
 #include "../../include/vox8.h"
+#include "vox8_fast.h"
 #include "vox8_kernels.h"
+#include "vox8_mg.h"
 #include "vox8_pcg.h"
 #include "vox8_topopt.h"
 
@@ -22,6 +24,10 @@ struct vox8_ctx {
   std::vector<void*> dev_allocs;
   std::vector<void*> host_allocs;
   double Ke[576]{}, Me[576]{};
+  int pattern = PAT_DENSE;          // structure of Ke/Me found at set time
+  int stencil_kind = 0;             // 0 auto, 1 force the dense 24x24 kernel
+  FastCoef<double> c64{};
+  FastCoef<float> c32{};
   bool have_mats = false, have_topo = false, have_density = false;
   double *dKe = nullptr, *dMe = nullptr;
   float *dKe32 = nullptr, *dMe32 = nullptr;
@@ -52,6 +58,10 @@ struct vox8_ctx {
   uint8_t* d_fmask = nullptr;
   double* d_den = nullptr;
   bool have_filter = false;
+  // multigrid (built on first use; re-set up when the density changes)
+  MultiGrid mg;
+  std::vector<uint8_t> h_active, h_fixed;
+  uint64_t density_version = 0, mg_version = ~0ull;
   // OC / sensitivities scratch
   double* d_oc = nullptr;
   double *d_kphi = nullptr, *d_mphi = nullptr, *d_ce = nullptr, *d_sums = nullptr,
@@ -118,8 +128,17 @@ int apply_op(vox8_ctx* c, int op, const T* x, int64_t ldx, T* yK, T* yM, int64_t
   } else {
     fK = (const T*)c->fK32; fM = (const T*)c->fM32; Ke = (const T*)c->dKe32; Me = (const T*)c->dMe32;
   }
-  VOX8_TRY(launch_apply_dense<T>(c->g, op, fK, fM, Ke, Me, c->d_fixed, x, ldx, yK, yM, ldy, nrhs,
-                                 c->st, fuse));
+  if (c->pattern != PAT_DENSE && c->stencil_kind == 0) {
+    if constexpr (sizeof(T) == 8)
+      VOX8_TRY(launch_apply_fast<double>(c->g, c->pattern, op, c->c64, fK, fM, c->d_fixed, x, ldx,
+                                         yK, yM, ldy, nrhs, c->st, fuse));
+    else
+      VOX8_TRY(launch_apply_fast<float>(c->g, c->pattern, op, c->c32, fK, fM, c->d_fixed, x, ldx,
+                                        yK, yM, ldy, nrhs, c->st, fuse));
+  } else {
+    VOX8_TRY(launch_apply_dense<T>(c->g, op, fK, fM, Ke, Me, c->d_fixed, x, ldx, yK, yM, ldy, nrhs,
+                                   c->st, fuse));
+  }
   if ((op & OP_M) && c->n_tip > 0)
     VOX8_TRY(launch_tip_mass<T>(c->d_tip, c->n_tip, c->tip_m, c->d_fixed, x, ldx, yM, ldy, nrhs, c->st));
   return OK;
@@ -172,6 +191,39 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   VOX8_TRY(halloc(c, &w.host, 32));
   w.kmax = k;
   w.n = n;
+  return OK;
+}
+
+int mg_dev_alloc(void* ctx, void** p, size_t bytes) {
+  uint8_t* q = nullptr;
+  VOX8_TRY(dalloc((vox8_ctx*)ctx, &q, bytes));
+  *p = q;
+  return OK;
+}
+int mg_pinned_alloc(void* ctx, void** p, size_t bytes) {
+  uint8_t* q = nullptr;
+  VOX8_TRY(halloc((vox8_ctx*)ctx, &q, bytes));
+  *p = q;
+  return OK;
+}
+
+// Multigrid preconditioner for the current density, with work space for ncols.
+int ensure_mg(vox8_ctx* c, int ncols) {
+  if (c->pattern == PAT_DENSE || c->stencil_kind != 0) {
+    set_error("the multigrid preconditioner needs the structured stencil (box-element Ke/Me)");
+    return ERR_STATE;
+  }
+  MGAlloc al{c, mg_dev_alloc, mg_pinned_alloc};
+  if (!c->mg.built) {
+    VOX8_TRY(mg_build(c->mg, c->g, c->h_active.data(), c->h_fixed.data(), c->Ke, c->pattern, c->c32,
+                      c->fK32, c->d_active, c->d_fixed, al));
+    c->mg_version = ~0ull;
+  }
+  VOX8_TRY(mg_reserve(c->mg, ncols, al));
+  if (c->mg_version != c->density_version) {
+    VOX8_TRY(mg_update(c->mg, c->fK, c->dKe, c->st));
+    c->mg_version = c->density_version;
+  }
   return OK;
 }
 
@@ -272,10 +324,57 @@ int vox8_set_element_matrices(vox8_ctx* c, const double* h_Ke, const double* h_M
   VOX8_HIP(hipMemcpyAsync(c->dKe32, k32, sizeof(k32), hipMemcpyHostToDevice, c->st));
   VOX8_HIP(hipMemcpyAsync(c->dMe32, m32, sizeof(m32), hipMemcpyHostToDevice, c->st));
   VOX8_HIP(hipStreamSynchronize(c->st));
+  c->pattern = fast_coefficients(c->Ke, c->Me, &c->c64, &c->c32);
+  c->mg.built = false;             // level operators follow Ke
   c->have_mats = true;
   c->dinv_valid = false;
   return OK;
   API_END
+}
+
+int vox8_set_stencil(vox8_ctx* c, int kind, int* h_pattern) {
+  VOX8_CHECK(c != nullptr, "null context");
+  VOX8_CHECK(kind == 0 || kind == 1, "kind must be 0 (auto) or 1 (dense)");
+  c->stencil_kind = kind;
+  c->mg.built = false;
+  if (h_pattern) *h_pattern = (kind == 1) ? PAT_DENSE : c->pattern;
+  return OK;
+}
+
+int vox8_precond_apply(vox8_ctx* c, const double* d_r, double* d_z, int nrhs) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_r && d_z && nrhs >= 1 && nrhs <= 64, "bad argument");
+  VOX8_TRY(ensure_mg(c, nrhs));
+  return mg_apply(c->mg, d_r, d_z, nrhs, c->st);
+  API_END
+}
+
+int vox8_mg_info(vox8_ctx* c, int* h_nlevels, int* h_dims, double* h_lam_max, int max_levels) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(h_nlevels != nullptr, "null output");
+  VOX8_TRY(ensure_mg(c, 1));
+  const int nl = (int)c->mg.lv.size();
+  *h_nlevels = nl;
+  for (int l = 0; l < nl && l < max_levels; ++l) {
+    if (h_dims) { h_dims[3 * l] = c->mg.lv[l].g.nx; h_dims[3 * l + 1] = c->mg.lv[l].g.ny; h_dims[3 * l + 2] = c->mg.lv[l].g.nz; }
+    if (h_lam_max) h_lam_max[l] = c->mg.lv[l].lam_max;
+  }
+  return OK;
+  API_END
+}
+
+int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs) {
+  VOX8_CHECK(c != nullptr, "null context");
+  VOX8_CHECK(nu >= 1 && nu <= 16 && alpha > 1.0 && coarse_dofs >= 24 && coarse_dofs <= 1000,
+             "mg options out of range (nu 1..16, alpha > 1, coarse_dofs 24..1000)");
+  c->mg.nu = nu;
+  c->mg.alpha = alpha;
+  if (c->mg.coarse_dofs != coarse_dofs) c->mg.built = false;
+  c->mg.coarse_dofs = coarse_dofs;
+  c->mg_version = ~0ull;
+  return OK;
 }
 
 int vox8_set_topology(vox8_ctx* c, const uint8_t* h_active, const uint8_t* h_fixed) {
@@ -285,6 +384,9 @@ int vox8_set_topology(vox8_ctx* c, const uint8_t* h_active, const uint8_t* h_fix
   VOX8_HIP(hipMemcpyAsync(c->d_active, h_active, c->g.n_elem, hipMemcpyHostToDevice, c->st));
   VOX8_HIP(hipMemcpyAsync(c->d_fixed, h_fixed, c->g.n_nodes, hipMemcpyHostToDevice, c->st));
   VOX8_HIP(hipStreamSynchronize(c->st));
+  c->h_active.assign(h_active, h_active + c->g.n_elem);
+  c->h_fixed.assign(h_fixed, h_fixed + c->g.n_nodes);
+  c->mg.built = false;             // level hierarchy follows the topology
   c->have_topo = true;
   c->have_density = false;
   c->dinv_valid = false;
@@ -323,6 +425,7 @@ int vox8_set_density(vox8_ctx* c, const double* d_rho, double p, double e_min, d
                                   c->fK32, c->fM32, c->st));
   c->have_density = true;
   c->dinv_valid = false;
+  ++c->density_version;
   return OK;
   API_END
 }
@@ -374,15 +477,12 @@ int vox8_pcg(vox8_ctx* c, const double* d_F, double* d_U, int nrhs, int64_t ld, 
   A64.apply = [c](const double* x, int64_t ldx, double* y, int64_t ldy, int nr, hipStream_t,
                   DotFuse f) { return apply_op<double>(c, OP_K, x, ldx, y, nullptr, ldy, nr, f); };
   if (prec == VOX8_PREC_MIXED) {
-    VOX8_TRY(ensure_pcg_work(c, c->w32, nrhs));
-    PcgOps<float> A32;
-    A32.g = &c->g;
-    A32.fixed = c->d_fixed;
-    A32.dinv = c->dinvK32;
-    A32.apply = [c](const float* x, int64_t ldx, float* y, int64_t ldy, int nr, hipStream_t,
-                    DotFuse f) { return apply_op<float>(c, OP_K, x, ldx, y, nullptr, ldy, nr, f); };
-    VOX8_TRY(pcg_mixed(A64, A32, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, 1e-4, c->w64,
-                       c->w32, c->st, res.data()));
+    VOX8_TRY(ensure_mg(c, nrhs));
+    Precond<double> M = [c](const double* r, double* z, int nr, hipStream_t st) {
+      return mg_apply(c->mg, r, z, nr, st);
+    };
+    VOX8_TRY(pcg_fp64(A64, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, c->w64, c->st,
+                      res.data(), &M));
   } else {
     VOX8_CHECK(prec == VOX8_PREC_FP64, "prec must be FP64 (0) or MIXED (1)");
     VOX8_TRY(pcg_fp64(A64, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, c->w64, c->st,
@@ -416,7 +516,15 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
     return apply_op<double>(c, OP_KM, x, c->g.n_dofs, yK, yM, c->g.n_dofs, nr, DotFuse());
   };
   LobpcgResult r{};
-  VOX8_TRY(lobpcg_fp64(A, km, k, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed));
+  if (prec == VOX8_PREC_MIXED) {
+    VOX8_TRY(ensure_mg(c, k));
+    Precond<double> M = [c](const double* rr, double* z, int nr, hipStream_t st) {
+      return mg_apply(c->mg, rr, z, nr, st);
+    };
+    VOX8_TRY(lobpcg_fp64(A, km, k, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed, &M));
+  } else {
+    VOX8_TRY(lobpcg_fp64(A, km, k, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed));
+  }
   if (h_iters) *h_iters = r.iters;
   if (h_info) *h_info = r.info;
   if (h_max_res) *h_max_res = r.max_res;

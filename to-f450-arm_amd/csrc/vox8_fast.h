@@ -1,0 +1,57 @@
+// vox8 — structured (Hadamard-basis) stencil: coefficient block and launcher.
+#pragma once
+#include "vox8_kernels.h"
+
+namespace vox8 {
+
+enum Pattern : int { PAT_DENSE = -1, PAT_ORTHO = 0, PAT_BLOCK = 1 };
+
+// Element operator in the monomial basis, scaled by 1/64 (see vox8_stencil_fast.hip):
+// lin: {xi, eta, zeta} x {x,y,z}, bil: {xi eta, eta zeta, xi zeta} x {x,y,z},
+// tri: {xi eta zeta} x {x,y,z}; md: mass diagonal per monomial (same for x,y,z).
+template <typename T>
+struct FastCoef {
+  T lin[9][9];
+  T bil[9][9];
+  T tri[3][3];
+  T md[8];
+};
+
+// Index pairs (row, col) of the orthotropic non-zero pattern.
+constexpr int ORTHO_LIN[21][2] = {{0, 0}, {0, 4}, {0, 8}, {1, 1}, {1, 3}, {2, 2}, {2, 6},
+                                  {3, 1}, {3, 3}, {4, 0}, {4, 4}, {4, 8}, {5, 5}, {5, 7},
+                                  {6, 2}, {6, 6}, {7, 5}, {7, 7}, {8, 0}, {8, 4}, {8, 8}};
+constexpr int ORTHO_BIL[21][2] = {{0, 0}, {0, 5}, {1, 1}, {1, 8}, {2, 2}, {2, 3}, {2, 7},
+                                  {3, 2}, {3, 3}, {3, 7}, {4, 4}, {4, 6}, {5, 0}, {5, 5},
+                                  {6, 4}, {6, 6}, {7, 2}, {7, 3}, {7, 7}, {8, 1}, {8, 8}};
+
+// Classify Ke/Me (24x24 row-major, reference local node order) and derive the
+// coefficients.  Returns PAT_ORTHO, PAT_BLOCK or PAT_DENSE.
+int fast_coefficients(const double* Ke, const double* Me, FastCoef<double>* c64,
+                      FastCoef<float>* c32);
+
+enum EpiMode : int { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB = 2 };
+
+// Node-local epilogue of a K product (multigrid smoothing), vectors with ld.
+template <typename T>
+struct Epilogue {
+  int mode = EPI_PLAIN;
+  const T* b = nullptr;      // RESID: right-hand side
+  T* r = nullptr;            // RESID: out; CHEB: in/out residual
+  T* xs = nullptr;           // CHEB: iterate (+= d)
+  T* dnew = nullptr;         // out: next direction
+  const T* dinv = nullptr;   // Jacobi inverse diagonal (one vector, no ld)
+  T c1 = 0, c2 = 0;
+  int64_t ld = 0;
+};
+
+int fast_zchunk(const Grid& g);
+int fast_blocks(const Grid& g);
+
+template <typename T>
+int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,
+                      const T* fM, const uint8_t* fixed, const T* x, int64_t ldx, T* yK, T* yM,
+                      int64_t ldy, int nrhs, hipStream_t st, DotFuse fuse = DotFuse(),
+                      Epilogue<T> ep = Epilogue<T>());
+
+}  // namespace vox8

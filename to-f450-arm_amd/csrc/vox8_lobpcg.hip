@@ -16,6 +16,8 @@
 //     round-robin parallel ordering (Golub & Van Loan, sec. 8.5);
 //   * the Ritz update, residual, its norms and the preconditioned residual
 //     are one pass over the blocks.
+#include <cstdlib>
+
 #include "vox8_kernels.h"
 #include "vox8_pcg.h"
 
@@ -108,44 +110,40 @@ __global__ void k_gram_finish(int chunks, int m, const double* __restrict__ part
   }
 }
 
-// ---- one-workgroup dense symmetric eigen-solver (cyclic Jacobi) -------------
-// A (m x m, row stride MMAX) is diagonalised in place; V accumulates the rotations.
+// ---- one-wavefront dense symmetric eigen-solver (cyclic Jacobi) ------------
+// A (m x m, row stride MMAX, in LDS) is diagonalised in place; V accumulates
+// the rotations.  Parallel ordering: the circle method pairs all indices into
+// disjoint rotations per round (Golub & Van Loan, sec. 8.5.5).  Launched with
+// one 64-wide wavefront, so the barriers are cheap.
 __device__ void jacobi_eig(double* A, double* V, int m) {
   __shared__ double rc[MMAX / 2 + 1], rs[MMAX / 2 + 1];
   __shared__ int rp[MMAX / 2 + 1], rq[MMAX / 2 + 1];
-  __shared__ double s_off, s_fro;
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int e = tid; e < m * m; e += nth) V[(e / m) * MMAX + e % m] = (e / m == e % m) ? 1.0 : 0.0;
   const int mm = (m & 1) ? m + 1 : m;      // odd m: one idle "bye" slot
   const int npair = mm / 2;
   __syncthreads();
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    if (tid == 0) {
-      double off = 0.0, fro = 0.0;
-      for (int i = 0; i < m; ++i)
-        for (int j = 0; j < m; ++j) {
-          const double a = A[i * MMAX + j];
-          fro += a * a;
-          if (i != j) off += a * a;
-        }
-      s_off = off;
-      s_fro = fro;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    double off = 0.0, fro = 0.0;
+    for (int e = tid; e < m * m; e += nth) {
+      const double a = A[(e / m) * MMAX + e % m];
+      fro += a * a;
+      if (e / m != e % m) off += a * a;
     }
-    __syncthreads();
-    if (s_off <= 1e-30 * s_fro || s_off == 0.0) break;
+    off = wave_sum(off);
+    fro = wave_sum(fro);
+    if (off <= 1e-30 * fro || off == 0.0) break;
     for (int round = 0; round < mm - 1; ++round) {
       if (tid < npair) {
-        // circle method: slot 0 fixed, slots 1..mm-1 rotate by `round`
         const int i = tid;
         const int a = (i == 0) ? 0 : 1 + (i - 1 + round) % (mm - 1);
-        const int b = 1 + (mm - 1 - 1 - i + round) % (mm - 1);
+        const int b = 1 + (mm - 2 - i + round) % (mm - 1);
         int p = a < b ? a : b, q = a < b ? b : a;
         double c = 1.0, s = 0.0;
         if (q < m) {
           const double apq = A[p * MMAX + q];
           const double app = A[p * MMAX + p], aqq = A[q * MMAX + q];
-          if (apq != 0.0 && fabs(apq) > 1e-300 &&
-              fabs(apq) > 1e-18 * sqrt(fabs(app) * fabs(aqq))) {
+          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app) * fabs(aqq))) {
             const double tau = (aqq - app) / (2.0 * apq);
             const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
                                         : -1.0 / (-tau + sqrt(1.0 + tau * tau));
@@ -158,8 +156,7 @@ __device__ void jacobi_eig(double* A, double* V, int m) {
         rc[i] = c; rs[i] = s; rp[i] = p; rq[i] = q;
       }
       __syncthreads();
-      // rows: A <- J^T A
-      for (int e = tid; e < npair * m; e += nth) {
+      for (int e = tid; e < npair * m; e += nth) {          // rows: A <- J^T A
         const int i = e / m, j = e % m;
         const int p = rp[i], q = rq[i];
         if (p < 0 || rs[i] == 0.0) continue;
@@ -169,8 +166,7 @@ __device__ void jacobi_eig(double* A, double* V, int m) {
         A[q * MMAX + j] = s * ap + c * aq;
       }
       __syncthreads();
-      // columns: A <- A J, V <- V J
-      for (int e = tid; e < npair * m; e += nth) {
+      for (int e = tid; e < npair * m; e += nth) {          // columns: A <- A J, V <- V J
         const int i = e / m, j = e % m;
         const int p = rp[i], q = rq[i];
         if (p < 0 || rs[i] == 0.0) continue;
@@ -188,20 +184,25 @@ __device__ void jacobi_eig(double* A, double* V, int m) {
   __syncthreads();
 }
 
-// Generalised Rayleigh-Ritz: GA c = lam GB c, k smallest.  Out (global):
-// C (m x k, row-major), lam (k), status (0 ok, 1 rank < k).
-__global__ void __launch_bounds__(256)
+// Generalised Rayleigh-Ritz: GA c = lam GB c, k smallest lam.  The basis is
+// orthonormalised in the K metric (GA), which separates directions the mass
+// metric cannot (high-frequency content has large K-norm, small M-norm):
+//   D = diag(GA)^-1/2, GA_s = D GA D = V diag(s) V^T, keep s > drop * max(s),
+//   Q = V_keep diag(s^-1/2), H = Q^T (D GB D) Q = Y diag(mu) Y^T,
+//   lam = 1/mu for the k largest mu, c = D Q y / sqrt(mu)  (M-normalised).
+// Out (global): C (m x k, row-major), lam (k), status (0 ok, 1 failure).
+__global__ void __launch_bounds__(64)
 k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__ C,
                 double* __restrict__ lam, double* __restrict__ status, double drop_tol) {
   __shared__ double A[MMAX * MMAX], B[MMAX * MMAX], V[MMAX * MMAX], T1[MMAX * MMAX];
-  __shared__ double dsc[MMAX], sig[MMAX], th[MMAX];
+  __shared__ double dsc[MMAX], sig[MMAX], mu[MMAX];
   __shared__ int keep[MMAX], perm[MMAX];
   __shared__ int s_r;
   const int tid = threadIdx.x, nth = blockDim.x;
   const double* GA = G;
   const double* GB = G + m * m;
   for (int i = tid; i < m; i += nth) {
-    const double d = GB[i * m + i];
+    const double d = GA[i * m + i];
     dsc[i] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
   }
   __syncthreads();
@@ -212,33 +213,30 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
     B[i * MMAX + j] = 0.5 * (GB[i * m + j] + GB[j * m + i]) * sc;
   }
   __syncthreads();
-  jacobi_eig(B, V, m);
+  jacobi_eig(A, V, m);
   if (tid == 0) {
     double smax = 0.0;
-    for (int i = 0; i < m; ++i) { sig[i] = B[i * MMAX + i]; smax = fmax(smax, sig[i]); }
+    for (int i = 0; i < m; ++i) { sig[i] = A[i * MMAX + i]; smax = fmax(smax, sig[i]); }
     int r = 0;
     for (int i = 0; i < m; ++i)
-      if (sig[i] > drop_tol * smax && dsc[i] >= 0.0) keep[r++] = i;
+      if (sig[i] > drop_tol * smax) keep[r++] = i;
     s_r = r;
   }
   __syncthreads();
   const int r = s_r;
-  // Q = V(:,keep) diag(sig^-1/2) -> stored in T1 (m x r)
-  for (int e = tid; e < m * r; e += nth) {
+  for (int e = tid; e < m * r; e += nth) {                 // Q (m x r) in T1
     const int i = e / r, l = e % r;
     T1[i * MMAX + l] = V[i * MMAX + keep[l]] / sqrt(sig[keep[l]]);
   }
   __syncthreads();
-  // V <- A Q  (m x r)
-  for (int e = tid; e < m * r; e += nth) {
+  for (int e = tid; e < m * r; e += nth) {                 // V <- B_s Q
     const int i = e / r, l = e % r;
     double s = 0.0;
-    for (int j = 0; j < m; ++j) s += A[i * MMAX + j] * T1[j * MMAX + l];
+    for (int j = 0; j < m; ++j) s += B[i * MMAX + j] * T1[j * MMAX + l];
     V[i * MMAX + l] = s;
   }
   __syncthreads();
-  // B <- Q^T (A Q)  (r x r), symmetrised
-  for (int e = tid; e < r * r; e += nth) {
+  for (int e = tid; e < r * r; e += nth) {                 // A <- Q^T B_s Q
     const int l = e / r, q = e % r;
     double s = 0.0;
     for (int i = 0; i < m; ++i) s += T1[i * MMAX + l] * V[i * MMAX + q];
@@ -250,78 +248,116 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
     B[l * MMAX + q] = 0.5 * (A[l * MMAX + q] + A[q * MMAX + l]);
   }
   __syncthreads();
-  jacobi_eig(B, V, r);          // V: eigenvectors Y (r x r)
+  jacobi_eig(B, V, r);                                      // V: eigenvectors Y (r x r)
   if (tid == 0) {
-    for (int i = 0; i < r; ++i) { th[i] = B[i * MMAX + i]; perm[i] = i; }
-    for (int i = 0; i < r; ++i) {          // selection sort ascending
+    for (int i = 0; i < r; ++i) { mu[i] = B[i * MMAX + i]; perm[i] = i; }
+    for (int i = 0; i < r; ++i) {                           // selection sort, descending mu
       int b = i;
       for (int j = i + 1; j < r; ++j)
-        if (th[perm[j]] < th[perm[b]]) b = j;
+        if (mu[perm[j]] > mu[perm[b]]) b = j;
       const int t = perm[i]; perm[i] = perm[b]; perm[b] = t;
     }
-    status[0] = r < k ? 1.0 : 0.0;
+    bool bad = r < k;
+    for (int j = 0; j < k && j < r; ++j) bad = bad || !(mu[perm[j]] > 0.0);
+    status[0] = bad ? 1.0 : 0.0;
   }
   __syncthreads();
   const int kk = r < k ? r : k;
-  // C = D^-1/2 Q Y(:, perm[:k])
   for (int e = tid; e < m * k; e += nth) {
     const int i = e / k, j = e % k;
     double s = 0.0;
-    if (j < kk)
+    if (j < kk && mu[perm[j]] > 0.0) {
       for (int l = 0; l < r; ++l) s += T1[i * MMAX + l] * V[l * MMAX + perm[j]];
-    C[i * k + j] = dsc[i] * s;
+      s *= dsc[i] / sqrt(mu[perm[j]]);
+    }
+    C[i * k + j] = s;
   }
-  for (int j = tid; j < k; j += nth) lam[j] = j < kk ? th[perm[j]] : 0.0;
+  for (int j = tid; j < k; j += nth) lam[j] = (j < kk && mu[perm[j]] > 0.0) ? 1.0 / mu[perm[j]] : 0.0;
 }
 
-// ---- Ritz update + residual + preconditioner, one pass ----------------------
-// S/AS/BS: column-major blocks (ld n) holding [X | W | P]; the first m columns
-// form the current basis.  Writes X', AX', BX' into the X slots, and when
-// with_p the new directions P' = [W P] C[k:] into the P slots, then
-// R = AX' - BX' diag(lam), W' = D^-1 R and reduces ||R_j||^2.
+// ---- Ritz update (templated on the block size) ------------------------------
+// S/AS/BS: column-major blocks (ld n) holding [X | W | P]; the first NB*K
+// columns form the current basis.  Writes X' = S C into the X slots of S and,
+// for NB > 1, the new directions P' = [W P] C[K:] into the P slots of S, AS
+// and BS.  AX' and BX' are not formed here: they are recomputed exactly by a
+// stencil pass (implicit updates drift by ~eps * lambda_max per iteration,
+// which swamps the residual of the lowest modes).
+template <int K, int NB>
 __global__ void __launch_bounds__(256)
-k_ritz_update(int64_t n, int m, int k, int with_p, double* __restrict__ S,
-              double* __restrict__ AS, double* __restrict__ BS, const double* __restrict__ Cg,
-              const double* __restrict__ lamg, const double* __restrict__ dinv,
-              double* __restrict__ partials, unsigned int* __restrict__ counter,
-              double* __restrict__ res_out) {
-  __shared__ double C[MMAX * 12], lam[12], sh[8], bs[12];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < m * k; e += blockDim.x) C[e] = Cg[e];
-  for (int j = tid; j < k; j += blockDim.x) lam[j] = lamg[j];
+k_ritz_combine(int64_t n, double* __restrict__ S, double* __restrict__ AS,
+               double* __restrict__ BS, const double* __restrict__ Cg) {
+  constexpr int M = NB * K;
+  __shared__ double C[M * K];
+  for (int e = threadIdx.x; e < M * K; e += blockDim.x) C[e] = Cg[e];
   __syncthreads();
-  double nrm[12];
-  for (int j = 0; j < 12; ++j) nrm[j] = 0.0;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + tid; r < n;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
-    double ax[12];
-    double v[MMAX];
-    for (int blk = 0; blk < 3; ++blk) {
-      double* Z = blk == 0 ? S : (blk == 1 ? AS : BS);
-      for (int i = 0; i < m; ++i) v[i] = Z[(int64_t)i * n + r];
-      for (int j = 0; j < k; ++j) {
-        double x = 0.0, p = 0.0;
-        for (int i = 0; i < k; ++i) x += v[i] * C[i * k + j];
-        for (int i = k; i < m; ++i) p += v[i] * C[i * k + j];
-        Z[(int64_t)j * n + r] = x + p;
-        if (with_p) Z[(int64_t)(2 * k + j) * n + r] = p;
-        if (blk == 1) ax[j] = x + p;
-        if (blk == 2) {
-          const double res = ax[j] - lam[j] * (x + p);
-          nrm[j] += res * res;
-          S[(int64_t)(k + j) * n + r] = dinv[r] * res;
+    double v[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) v[i] = S[(int64_t)i * n + r];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      double x = 0.0, p = 0.0;
+#pragma unroll
+      for (int i = 0; i < K; ++i) x += v[i] * C[i * K + j];
+#pragma unroll
+      for (int i = K; i < M; ++i) p += v[i] * C[i * K + j];
+      S[(int64_t)j * n + r] = x + p;
+      if (NB > 1) S[(int64_t)(2 * K + j) * n + r] = p;
+    }
+    if (NB > 1) {
+#pragma unroll
+      for (int blk = 0; blk < 2; ++blk) {
+        double* Z = blk == 0 ? AS : BS;
+        double w[M > K ? M - K : 1];
+#pragma unroll
+        for (int i = K; i < M; ++i) w[i - K] = Z[(int64_t)i * n + r];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          double p = 0.0;
+#pragma unroll
+          for (int i = K; i < M; ++i) p += w[i - K] * C[i * K + j];
+          Z[(int64_t)(2 * K + j) * n + r] = p;
         }
       }
     }
   }
-  for (int j = 0; j < k; ++j) {
+}
+
+// R = AX - BX diag(lam) (exact products), ||R_j||^2 reduced, W = D^-1 R
+// (or W = R when a preconditioner follows).
+template <int K>
+__global__ void __launch_bounds__(256)
+k_residual(int64_t n, double* __restrict__ S, const double* __restrict__ AS,
+           const double* __restrict__ BS, const double* __restrict__ lamg,
+           const double* __restrict__ dinv, double* __restrict__ partials,
+           unsigned int* __restrict__ counter, double* __restrict__ res_out) {
+  __shared__ double lam[K], sh[8], bs[K];
+  __shared__ int s_last;
+  const int tid = threadIdx.x;
+  if (tid < K) lam[tid] = lamg[tid];
+  __syncthreads();
+  double nrm[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) nrm[j] = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + tid; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const double di = dinv ? dinv[r] : 1.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const double res = AS[(int64_t)j * n + r] - lam[j] * BS[(int64_t)j * n + r];
+      nrm[j] += res * res;
+      S[(int64_t)(K + j) * n + r] = di * res;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
     const double t = block_sum(nrm[j], sh);
     if (tid == 0) bs[j] = t;
   }
   __syncthreads();
-  __shared__ int s_last;
   if (tid == 0) {
-    for (int j = 0; j < k; ++j)
+    for (int j = 0; j < K; ++j)
       __hip_atomic_store(&partials[(int64_t)blockIdx.x * 12 + j], bs[j], __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
@@ -330,7 +366,7 @@ k_ritz_update(int64_t n, int m, int k, int with_p, double* __restrict__ S,
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  for (int j = 0; j < k; ++j) {
+  for (int j = 0; j < K; ++j) {
     double t = 0.0;
     for (int b = tid; b < (int)gridDim.x; b += blockDim.x)
       t += __hip_atomic_load(&partials[(int64_t)b * 12 + j], __ATOMIC_RELAXED,
@@ -364,18 +400,47 @@ static int gram_and_rr(LobpcgWork& w, int m, int k, hipStream_t st, double drop_
   double* C = w.G + 2 * MMAX * MMAX;
   double* lam = C + MMAX * 12;
   double* status = lam + 12;
-  hipLaunchKernelGGL(k_rayleigh_ritz, dim3(1), dim3(256), 0, st, m, k, (const double*)w.G, C, lam,
+  hipLaunchKernelGGL(k_rayleigh_ritz, dim3(1), dim3(64), 0, st, m, k, (const double*)w.G, C, lam,
                      status, drop_tol);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
 
-static int ritz_update(LobpcgWork& w, int m, int k, int with_p, const double* dinv, hipStream_t st) {
-  double* C = w.G + 2 * MMAX * MMAX;
-  double* lam = C + MMAX * 12;
-  hipLaunchKernelGGL(k_ritz_update, dim3(grid_blocks(w.n)), dim3(256), 0, st, w.n, m, k, with_p,
-                     w.S, w.AS, w.BS, (const double*)C, (const double*)lam, dinv,
-                     w.res_partials, w.counters, w.res);
+template <int K>
+static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStream_t st) {
+  const dim3 g(grid_blocks(w.n)), b(256);
+  if (nb == 1) hipLaunchKernelGGL((k_ritz_combine<K, 1>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
+  else if (nb == 2) hipLaunchKernelGGL((k_ritz_combine<K, 2>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
+  else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
+}
+
+template <int K>
+static void launch_residual(const LobpcgWork& w, const double* lam, const double* dinv, hipStream_t st) {
+  hipLaunchKernelGGL((k_residual<K>), dim3(grid_blocks(w.n)), dim3(256), 0, st, w.n, w.S,
+                     (const double*)w.AS, (const double*)w.BS, lam, dinv, w.res_partials, w.counters,
+                     w.res);
+}
+
+#define VOX8_K_SWITCH(k, F, ...)                                                              \
+  switch (k) {                                                                              \
+    case 1: F<1>(__VA_ARGS__); break;  case 2: F<2>(__VA_ARGS__); break;                    \
+    case 3: F<3>(__VA_ARGS__); break;  case 4: F<4>(__VA_ARGS__); break;                    \
+    case 5: F<5>(__VA_ARGS__); break;  case 6: F<6>(__VA_ARGS__); break;                    \
+    case 7: F<7>(__VA_ARGS__); break;  case 8: F<8>(__VA_ARGS__); break;                    \
+    case 9: F<9>(__VA_ARGS__); break;  case 10: F<10>(__VA_ARGS__); break;                  \
+    case 11: F<11>(__VA_ARGS__); break; default: F<12>(__VA_ARGS__); break;                 \
+  }
+
+// X' (and P') from the Ritz coefficients, exact AX'/BX' by the stencil, residual.
+static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
+                     const std::function<int(const double*, double*, double*, int, hipStream_t)>& applyKM,
+                     hipStream_t st) {
+  const double* C = w.G + 2 * MMAX * MMAX;
+  const double* lam = C + MMAX * 12;
+  VOX8_K_SWITCH(k, launch_combine, w, nb, C, st);
+  VOX8_HIP(hipGetLastError());
+  VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st));
+  VOX8_K_SWITCH(k, launch_residual, w, lam, dinv, st);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -393,14 +458,18 @@ static int read_state(LobpcgWork& w, int k, hipStream_t st) {
 int lobpcg_fp64(const PcgOps<double>& A,
                 const std::function<int(const double*, double*, double*, int, hipStream_t)>& applyKM,
                 int k, double tol, int maxiter, const double* X0, double* Xout, double* lam_out,
-                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed) {
+                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
+                const Precond<double>* M) {
   VOX8_CHECK(k >= 1 && k <= 12, "lobpcg: block size must be 1..12");
   const int64_t n = w.n;
   double* X = w.S;
   double* W = w.S + (int64_t)k * n;
   double* AW = w.AS + (int64_t)k * n;
   double* BW = w.BS + (int64_t)k * n;
-  const double drop_tol = 1e-12;
+  // deflation: drop basis directions whose scaled K-Gram eigenvalue is below
+  // drop_tol * max (numerically dependent in the energy norm)
+  double drop_tol = 1e-13;
+  if (const char* e = getenv("VOX8_LOBPCG_DROP")) drop_tol = atof(e);
   if (X0) {
     hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * k)), dim3(256), 0, st, n, k, X0, X);
     VOX8_HIP(hipGetLastError());
@@ -410,8 +479,10 @@ int lobpcg_fp64(const PcgOps<double>& A,
   }
   // initial Rayleigh-Ritz on X alone
   VOX8_TRY(applyKM(X, w.AS, w.BS, k, st));
+  const double* Tdinv = M ? nullptr : A.dinv;
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
-  VOX8_TRY(ritz_update(w, k, k, 0, A.dinv, st));
+  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st));
+  if (M) VOX8_TRY((*M)(W, W, k, st));
   VOX8_TRY(read_state(w, k, st));
   VOX8_CHECK(w.host[12] == 0.0, "lobpcg: linearly dependent initial block");
   std::vector<int> conv(k, 0);
@@ -437,16 +508,17 @@ int lobpcg_fp64(const PcgOps<double>& A,
       VOX8_TRY(read_state(w, k, st));
     }
     if (w.host[12] != 0.0) break;           // cannot continue: return the current block
-    VOX8_TRY(ritz_update(w, m, k, 1, A.dinv, st));
+    VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st));
+    if (M) VOX8_TRY((*M)(W, W, k, st));
     have_p = true;
     ++it;
     VOX8_TRY(read_state(w, k, st));
     all = update_conv();
   }
-  // final explicit Rayleigh-Ritz (scipy lobpcg post-processing)
-  VOX8_TRY(applyKM(X, w.AS, w.BS, k, st));
+  // final Rayleigh-Ritz on X alone (scipy lobpcg post-processing); AX, BX
+  // are exact products of the current X
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
-  VOX8_TRY(ritz_update(w, k, k, 0, A.dinv, st));
+  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st));
   VOX8_TRY(read_state(w, k, st));
   hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * k)), dim3(256), 0, st, n, k,
                      (const double*)X, Xout);

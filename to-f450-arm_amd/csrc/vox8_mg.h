@@ -1,0 +1,64 @@
+// vox8 — geometric multigrid preconditioner on the structured voxel grid.
+#pragma once
+#include <vector>
+
+#include "vox8_fast.h"
+
+namespace vox8 {
+
+struct MGLevel {
+  Grid g{};
+  double scale = 1.0;          // element matrix of this level = scale * Ke (box scaled by 2^l)
+  int pattern = PAT_DENSE;
+  FastCoef<float> c32{};
+  double diag24[24]{};         // diagonal of the level's element matrix
+  double* fK64 = nullptr;      // element factors (FP64 master copy)
+  float* fK = nullptr;         // element factors used by the FP32 smoother
+  uint8_t* active = nullptr;
+  uint8_t* fixed = nullptr;    // per-node fixed-DOF bitmask
+  float* dinv = nullptr;       // Jacobi inverse diagonal
+  double lam_max = 2.0;        // estimate of lambda_max(D^-1 K)
+  float* pv = nullptr;         // power-iteration vector, kept across density updates
+  bool pv_init = false;
+  // work vectors, ncols column-major blocks of n_dofs
+  float *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d2 = nullptr;
+};
+
+struct MultiGrid {
+  std::vector<MGLevel> lv;
+  int ncols = 0;               // capacity of the work vectors
+  int nu = 2;                  // Chebyshev smoothing steps before and after
+  int coarse_dofs = 700;       // stop coarsening at this many DOFs (dense inverse)
+  double alpha = 12.0;         // smoothing interval [lam_max/alpha, lam_max]
+  // coarsest level: dense inverse when small
+  int n_coarse = 0;
+  double* A = nullptr;         // n_coarse^2 FP64 (assembled, then inverted in place)
+  float* Ainv = nullptr;       // FP32 copy used in the V-cycle
+  double* red = nullptr;       // reduction scratch
+  unsigned int* counter = nullptr;
+  double* scal = nullptr;      // power-iteration scalars
+  double* host = nullptr;      // pinned
+  float* tmp32 = nullptr;      // fine-level input/output staging (ncols * n0)
+  bool built = false;
+  bool updated = false;
+};
+
+// Allocation helpers are supplied by the context (device/pinned, tracked).
+struct MGAlloc {
+  void* ctx;
+  int (*dev)(void* ctx, void** p, size_t bytes);
+  int (*pinned)(void* ctx, void** p, size_t bytes);
+};
+
+// Build the level hierarchy for the fine grid and its host masks.
+int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8_t* h_fixed,
+             const double* Ke, int pattern0, const FastCoef<float>& c0, float* fK0_32,
+             uint8_t* d_active0, uint8_t* d_fixed0, const MGAlloc& al);
+// Per-density setup: restrict factors, Jacobi diagonals, lambda_max, coarse inverse.
+int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t st);
+// Ensure work vectors for ncols columns.
+int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al);
+// z = V-cycle(r) for ncols FP64 column-major vectors (ld = fine n_dofs).
+int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st);
+
+}  // namespace vox8

@@ -1,0 +1,529 @@
+// vox8 — geometric multigrid V-cycle preconditioner for K on the voxel grid.
+//
+// The reference preconditions its CG and LOBPCG with the Jacobi diagonal only
+// (src/fem.py:441-443, 543-547; src/fem_gpu.py:61-62, 140-141).  Both solvers'
+// stopping tests are unchanged when a stronger preconditioner is used; only
+// the iteration count drops.  This V-cycle is a fixed symmetric operator:
+//   * levels: the box grid coarsened 2:1 in every axis (ceil for odd counts),
+//     coarse element = box of twice the size, so its element matrix is 2^l Ke
+//     (exact for any constant material), factor = mean of the 8 children
+//     (rediscretisation);
+//   * transfer: trilinear prolongation P, restriction R = P^T, both masked by
+//     the fixed DOFs of their level;
+//   * smoother: Chebyshev polynomial of D^-1 K on [lam_max/alpha, lam_max]
+//     (three-term recurrence, Saad, Iterative Methods, sec. 12.1), nu steps
+//     before and after; lam_max from power iteration, +10 % margin;
+//   * coarsest level: dense inverse (in-place Gauss-Jordan of the SPD matrix).
+// Smoothing and transfers run in FP32 on the structured stencil with fused
+// Chebyshev epilogues; the outer Krylov method stays FP64.
+#include <cmath>
+
+#include "vox8_mg.h"
+
+namespace vox8 {
+
+struct Diag24 { double v[24]; };
+
+static unsigned nb(int64_t n, int cap = 2048) {
+  int64_t b = (n + 255) / 256;
+  return (unsigned)(b < cap ? (b < 1 ? 1 : b) : cap);
+}
+
+__global__ void k_mg_factors(Grid gf, Grid gc, const double* __restrict__ ff,
+                             const uint8_t* __restrict__ af, double* __restrict__ fc64,
+                             float* __restrict__ fc32, uint8_t* __restrict__ ac) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= gc.n_elem) return;
+  const int X = (int)(e % gc.nx), Y = (int)((e / gc.nx) % gc.ny), Z = (int)(e / ((int64_t)gc.nx * gc.ny));
+  double s = 0.0;
+  uint8_t act = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
+    if (x < gf.nx && y < gf.ny && z < gf.nz) {
+      const int64_t f = x + (int64_t)gf.nx * (y + (int64_t)gf.ny * z);
+      s += ff[f];
+      act |= af[f];
+    }
+  }
+  fc64[e] = 0.125 * s;
+  fc32[e] = (float)(0.125 * s);
+  ac[e] = act;
+}
+
+__global__ void k_mg_jacobi(Grid g, const double* __restrict__ f, Diag24 dg,
+                            const uint8_t* __restrict__ fixed, float* __restrict__ dinv) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
+  double d[3] = {0, 0, 0};
+  for (int s = 0; s < 8; ++s) {
+    const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
+    const int ex = ix - 1 + sx, ey = iy - 1 + sy, ez = iz - 1 + sz;
+    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+    const double fe = f[ex + (int64_t)g.nx * (ey + (int64_t)g.ny * ez)];
+    const int a = local_idx(1 - sx, 1 - sy, 1 - sz);
+    for (int c = 0; c < 3; ++c) d[c] += fe * dg.v[3 * a + c];
+  }
+  const uint8_t fx = fixed[n];
+  for (int c = 0; c < 3; ++c)
+    dinv[3 * n + c] = ((fx >> c) & 1) ? 0.0f : (d[c] != 0.0 ? (float)(1.0 / d[c]) : 1.0f);
+}
+
+// r = b, d = s * dinv * b, x = 0
+__global__ void k_mg_cheb_init(int64_t n, const float* __restrict__ b, const float* __restrict__ dinv,
+                               float s, float* __restrict__ r, float* __restrict__ d,
+                               float* __restrict__ x) {
+  const int64_t off = (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = b[off + i];
+    r[off + i] = v;
+    d[off + i] = s * dinv[i] * v;
+    x[off + i] = 0.0f;
+  }
+}
+
+// coarse b = R r: weights 1 at the coincident fine node, 1/2 at the neighbours, per axis
+__global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
+                              const uint8_t* __restrict__ fixc, float* __restrict__ bc) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= gc.n_nodes) return;
+  const int col = blockIdx.y;
+  rf += (int64_t)col * gf.n_dofs;
+  bc += (int64_t)col * gc.n_dofs;
+  const int I = (int)(n % gc.NX), J = (int)((n / gc.NX) % gc.NY), K = (int)(n / ((int64_t)gc.NX * gc.NY));
+  float s[3] = {0.f, 0.f, 0.f};
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = 2 * K + dz;
+    if (z < 0 || z >= gf.NZ) continue;
+    const float wz = dz ? 0.5f : 1.0f;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = 2 * J + dy;
+      if (y < 0 || y >= gf.NY) continue;
+      const float wy = wz * (dy ? 0.5f : 1.0f);
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = 2 * I + dx;
+        if (x < 0 || x >= gf.NX) continue;
+        const float w = wy * (dx ? 0.5f : 1.0f);
+        const int64_t f = x + (int64_t)gf.NX * (y + (int64_t)gf.NY * z);
+        for (int c = 0; c < 3; ++c) s[c] += w * rf[3 * f + c];
+      }
+    }
+  }
+  const uint8_t fx = fixc[n];
+  for (int c = 0; c < 3; ++c) bc[3 * n + c] = ((fx >> c) & 1) ? 0.0f : s[c];
+}
+
+// fine x += P coarse x (trilinear), fixed fine DOFs untouched
+__global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
+                                 const uint8_t* __restrict__ fixf, float* __restrict__ xf) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= gf.n_nodes) return;
+  const int col = blockIdx.y;
+  xc += (int64_t)col * gc.n_dofs;
+  xf += (int64_t)col * gf.n_dofs;
+  const int i = (int)(n % gf.NX), j = (int)((n / gf.NX) % gf.NY), k = (int)(n / ((int64_t)gf.NX * gf.NY));
+  const int x0 = i >> 1, x1 = (i + 1) >> 1, y0 = j >> 1, y1 = (j + 1) >> 1, z0 = k >> 1, z1 = (k + 1) >> 1;
+  const float wx = (x0 == x1) ? 1.0f : 0.5f, wy = (y0 == y1) ? 1.0f : 0.5f, wz = (z0 == z1) ? 1.0f : 0.5f;
+  float s[3] = {0.f, 0.f, 0.f};
+  for (int cz = 0; cz < (z0 == z1 ? 1 : 2); ++cz)
+    for (int cy = 0; cy < (y0 == y1 ? 1 : 2); ++cy)
+      for (int cx = 0; cx < (x0 == x1 ? 1 : 2); ++cx) {
+        const int64_t c = (cx ? x1 : x0) + (int64_t)gc.NX * ((cy ? y1 : y0) + (int64_t)gc.NY * (cz ? z1 : z0));
+        for (int q = 0; q < 3; ++q) s[q] += xc[3 * c + q];
+      }
+  const float w = wx * wy * wz;
+  const uint8_t fx = fixf[n];
+  for (int q = 0; q < 3; ++q)
+    if (!((fx >> q) & 1)) xf[3 * n + q] += w * s[q];
+}
+
+// dense coarse matrix A[i][j] (identity rows/cols at fixed DOFs)
+__global__ void k_mg_coarse_assemble(Grid g, const double* __restrict__ f, const uint8_t* __restrict__ fixed,
+                                     const double* __restrict__ Ke, double scale, double* __restrict__ A) {
+  const int64_t n = g.n_dofs;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e % n;
+    const int64_t ni = i / 3, nj = j / 3;
+    const int ci = (int)(i % 3), cj = (int)(j % 3);
+    if (((fixed[ni] >> ci) & 1) || ((fixed[nj] >> cj) & 1)) { A[e] = (i == j) ? 1.0 : 0.0; continue; }
+    const int ix = (int)(ni % g.NX), iy = (int)((ni / g.NX) % g.NY), iz = (int)(ni / ((int64_t)g.NX * g.NY));
+    const int jx = (int)(nj % g.NX), jy = (int)((nj / g.NX) % g.NY), jz = (int)(nj / ((int64_t)g.NX * g.NY));
+    double s = 0.0;
+    for (int q = 0; q < 8; ++q) {
+      const int sx = q & 1, sy = (q >> 1) & 1, sz = q >> 2;
+      const int ex = ix - 1 + sx, ey = iy - 1 + sy, ez = iz - 1 + sz;
+      if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+      const int ox = jx - ex, oy = jy - ey, oz = jz - ez;
+      if (ox < 0 || ox > 1 || oy < 0 || oy > 1 || oz < 0 || oz > 1) continue;
+      const int a = local_idx(1 - sx, 1 - sy, 1 - sz), b = local_idx(ox, oy, oz);
+      s += f[ex + (int64_t)g.nx * (ey + (int64_t)g.ny * ez)] * Ke[(3 * a + ci) * 24 + 3 * b + cj];
+    }
+    A[e] = scale * s;
+  }
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one pivot
+// per launch, every entry of the n x n matrix updated in parallel:
+//   p = 1/a_kk; a_kj *= p (j != k); a_ij -= a_ik a_kj (i, j != k);
+//   a_ik = -a_ik p (i != k); a_kk = p.
+// The pivot row is read in its pre-step state from `rowk`, the pivot column
+// from `colk` (both copied by k_gj_stage), so the update has no read/write hazard.
+__global__ void k_gj_stage(int n, int k, const double* __restrict__ A, double* __restrict__ rowk,
+                           double* __restrict__ colk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  rowk[i] = A[(int64_t)k * n + i];
+  colk[i] = A[(int64_t)i * n + k];
+}
+
+__global__ void k_gj_step(int n, int k, double* __restrict__ A, const double* __restrict__ rowk,
+                          const double* __restrict__ colk) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  const double p = 1.0 / rowk[k];
+  double v;
+  if (i == k) v = (j == k) ? p : rowk[j] * p;
+  else if (j == k) v = -colk[i] * p;
+  else v = A[e] - colk[i] * rowk[j] * p;
+  A[e] = v;
+}
+
+__global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// x = Ainv b per column; Ainv symmetric, read column-wise (coalesced over rows);
+// 64 rows x 4 partial sums per block, combined in LDS.
+__global__ void __launch_bounds__(256)
+k_mg_coarse_apply(int n, const float* __restrict__ Ainv, const float* __restrict__ b,
+                  float* __restrict__ x) {
+  __shared__ float part[4][64];
+  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int grp = threadIdx.x >> 6;
+  const int col = blockIdx.y;
+  const float* bc = b + (int64_t)col * n;
+  float s = 0.0f;
+  if (row < n)
+    for (int j = grp; j < n; j += 4) s += Ainv[(int64_t)j * n + row] * bc[j];
+  part[grp][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (grp == 0 && row < n)
+    x[(int64_t)col * n + row] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                               part[3][threadIdx.x];
+}
+
+__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  const int64_t off = (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[off + i] = (float)a[off + i];
+}
+
+__global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+  const int64_t off = (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[off + i] = (double)a[off + i];
+}
+
+// power iteration helpers: v random; w = dinv*y, ||w||^2 -> scal[0]=1/||w||, scal[1]=||w||
+__global__ void k_pow_init(int64_t n, const uint8_t* __restrict__ fixed, float* __restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const float u = (float)((z >> 40) * (1.0 / 16777216.0)) - 0.5f;
+    v[i] = ((fixed[i / 3] >> (i % 3)) & 1) ? 0.0f : u;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_pow_scale(int64_t n, const float* __restrict__ dinv, float* __restrict__ w, double* __restrict__ part,
+            unsigned int* __restrict__ counter, double* __restrict__ scal) {
+  __shared__ double sh[8];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float v = dinv[i] * w[i];
+    w[i] = v;
+    s += (double)v * v;
+  }
+  const double t = block_sum(s, sh);
+  double tot;
+  if (grid_reduce_last<1>(&t, part, counter, gridDim.x, blockIdx.x, &tot, sh))
+    if (threadIdx.x == 0) { scal[0] = tot > 0 ? 1.0 / sqrt(tot) : 0.0; scal[1] = sqrt(tot); }
+}
+
+__global__ void k_pow_norm(int64_t n, const double* __restrict__ scal, const float* __restrict__ w,
+                           float* __restrict__ v) {
+  const float s = (float)scal[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = s * w[i];
+}
+
+// ---------------------------------------------------------------------------
+static int level_apply(const MGLevel& L, const float* x, float* y, int ncols, hipStream_t st,
+                       Epilogue<float> ep) {
+  return launch_apply_fast<float>(L.g, L.pattern, OP_K, L.c32, L.fK, nullptr, L.fixed, x,
+                                  L.g.n_dofs, y, nullptr, L.g.n_dofs, ncols, st, DotFuse(), ep);
+}
+
+int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8_t* h_fixed,
+             const double* Ke, int pattern0, const FastCoef<float>& c0, float* fK0_32,
+             uint8_t* d_active0, uint8_t* d_fixed0, const MGAlloc& al) {
+  VOX8_CHECK(pattern0 != PAT_DENSE, "multigrid needs the structured (box-element) stencil");
+  mg.lv.clear();
+  std::vector<uint8_t> act(h_active, h_active + g0.n_elem), fix(h_fixed, h_fixed + g0.n_nodes);
+  Grid g = g0;
+  double scale = 1.0;
+  for (int l = 0; l < 16; ++l) {
+    MGLevel L;
+    L.g = g;
+    L.scale = scale;
+    L.pattern = pattern0;
+    L.c32 = c0;
+    for (int r = 0; r < 9; ++r)
+      for (int q = 0; q < 9; ++q) { L.c32.lin[r][q] *= (float)scale; L.c32.bil[r][q] *= (float)scale; }
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) L.c32.tri[r][q] *= (float)scale;
+    for (int i = 0; i < 24; ++i) L.diag24[i] = scale * Ke[i * 25];
+    void* p = nullptr;
+    if (l == 0) {
+      L.fK = fK0_32;
+      L.active = d_active0;
+      L.fixed = d_fixed0;
+    } else {
+      VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * g.n_elem)); L.fK64 = (double*)p;
+      VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_elem)); L.fK = (float*)p;
+      VOX8_TRY(al.dev(al.ctx, &p, g.n_elem)); L.active = (uint8_t*)p;
+      VOX8_TRY(al.dev(al.ctx, &p, g.n_nodes)); L.fixed = (uint8_t*)p;
+      VOX8_HIP(hipMemcpy(L.active, act.data(), g.n_elem, hipMemcpyHostToDevice));
+      VOX8_HIP(hipMemcpy(L.fixed, fix.data(), g.n_nodes, hipMemcpyHostToDevice));
+    }
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.dinv = (float*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.pv = (float*)p;
+    mg.lv.push_back(L);
+    // coarsen until the grid is tiny (dense inverse is then cheap) or a side
+    // cannot be halved any more
+    const bool small = g.n_dofs <= mg.coarse_dofs;
+    if (small || g.nx < 2 || g.ny < 2 || g.nz < 2) break;
+    // next level: masks on the host
+    Grid gc = make_grid((g.nx + 1) / 2, (g.ny + 1) / 2, (g.nz + 1) / 2);
+    std::vector<uint8_t> actc(gc.n_elem, 0), fixc(gc.n_nodes, 0), used(gc.n_nodes, 0);
+    for (int Z = 0; Z < gc.nz; ++Z)
+      for (int Y = 0; Y < gc.ny; ++Y)
+        for (int X = 0; X < gc.nx; ++X) {
+          uint8_t a = 0;
+          for (int k = 0; k < 8; ++k) {
+            const int x = 2 * X + (k & 1), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k >> 2);
+            if (x < g.nx && y < g.ny && z < g.nz) a |= act[x + (int64_t)g.nx * (y + (int64_t)g.ny * z)];
+          }
+          actc[X + (int64_t)gc.nx * (Y + (int64_t)gc.ny * Z)] = a;
+          if (a)
+            for (int k = 0; k < 8; ++k)
+              used[(X + (k & 1)) + (int64_t)gc.NX * ((Y + ((k >> 1) & 1)) + (int64_t)gc.NY * (Z + (k >> 2)))] = 1;
+        }
+    for (int K = 0; K < gc.NZ; ++K)
+      for (int J = 0; J < gc.NY; ++J)
+        for (int I = 0; I < gc.NX; ++I) {
+          const int64_t c = I + (int64_t)gc.NX * (J + (int64_t)gc.NY * K);
+          uint8_t f = 0;
+          if (2 * I < g.NX && 2 * J < g.NY && 2 * K < g.NZ)
+            f = fix[2 * I + (int64_t)g.NX * (2 * J + (int64_t)g.NY * 2 * K)];
+          if (!used[c]) f = 7;
+          fixc[c] = f;
+        }
+    act.swap(actc);
+    fix.swap(fixc);
+    g = gc;
+    scale *= 2.0;
+  }
+  const MGLevel& C = mg.lv.back();
+  mg.n_coarse = C.g.n_dofs <= 700 ? (int)C.g.n_dofs : 0;
+  void* p = nullptr;
+  if (mg.n_coarse) {
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * mg.n_coarse * mg.n_coarse)); mg.A = (double*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * mg.n_coarse * mg.n_coarse)); mg.Ainv = (float*)p;
+  }
+  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 4096)); mg.red = (double*)p;
+  VOX8_TRY(al.dev(al.ctx, &p, sizeof(unsigned int) * 4)); mg.counter = (unsigned int*)p;
+  VOX8_HIP(hipMemset(mg.counter, 0, sizeof(unsigned int) * 4));
+  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 8)); mg.scal = (double*)p;
+  VOX8_TRY(al.pinned(al.ctx, &p, sizeof(double) * 8)); mg.host = (double*)p;
+  mg.ncols = 0;
+  mg.built = true;
+  mg.updated = false;
+  return OK;
+}
+
+int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al) {
+  if (mg.ncols >= ncols) return OK;
+  for (auto& L : mg.lv) {
+    void* p = nullptr;
+    const size_t bytes = sizeof(float) * (size_t)ncols * L.g.n_dofs;
+    VOX8_TRY(al.dev(al.ctx, &p, bytes)); L.x = (float*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, bytes)); L.r = (float*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, bytes)); L.b = (float*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, bytes)); L.d = (float*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, bytes)); L.d2 = (float*)p;
+  }
+  mg.ncols = ncols;
+  return OK;
+}
+
+int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t st) {
+  // level factors and Jacobi diagonals
+  for (size_t l = 0; l < mg.lv.size(); ++l) {
+    MGLevel& L = mg.lv[l];
+    const double* f64 = l == 0 ? fK0 : L.fK64;
+    if (l > 0) {
+      const MGLevel& F = mg.lv[l - 1];
+      hipLaunchKernelGGL(k_mg_factors, dim3(nb(L.g.n_elem, 1 << 20)), dim3(256), 0, st, F.g, L.g,
+                         (const double*)(l == 1 ? fK0 : F.fK64), (const uint8_t*)F.active, L.fK64, L.fK,
+                         L.active);
+      VOX8_HIP(hipGetLastError());
+    }
+    Diag24 dg;
+    for (int i = 0; i < 24; ++i) dg.v[i] = L.diag24[i];
+    hipLaunchKernelGGL(k_mg_jacobi, dim3(nb(L.g.n_nodes, 1 << 20)), dim3(256), 0, st, L.g, f64, dg,
+                       (const uint8_t*)L.fixed, L.dinv);
+    VOX8_HIP(hipGetLastError());
+  }
+  // lambda_max(D^-1 K) per smoothed level by power iteration.  The spectrum
+  // top is clustered, so a cold start needs ~40 steps to come within a few %
+  // (from below); later updates restart from the previous vector.
+  const size_t nsm = mg.n_coarse ? mg.lv.size() - 1 : mg.lv.size();
+  for (size_t l = 0; l < nsm; ++l) {
+    MGLevel& L = mg.lv[l];
+    const int64_t n = L.g.n_dofs;
+    int steps = 8;
+    if (!L.pv_init) {
+      hipLaunchKernelGGL(k_pow_init, dim3(nb(n)), dim3(256), 0, st, n, (const uint8_t*)L.fixed, L.pv);
+      L.pv_init = true;
+      steps = 40;
+    }
+    for (int it = 0; it < steps; ++it) {
+      VOX8_TRY(level_apply(L, L.pv, L.r, 1, st, Epilogue<float>()));
+      hipLaunchKernelGGL(k_pow_scale, dim3(nb(n, 1024)), dim3(256), 0, st, n, (const float*)L.dinv, L.r,
+                         mg.red, mg.counter, mg.scal);
+      hipLaunchKernelGGL(k_pow_norm, dim3(nb(n)), dim3(256), 0, st, n, (const double*)mg.scal,
+                         (const float*)L.r, L.pv);
+    }
+    VOX8_HIP(hipGetLastError());
+    VOX8_HIP(hipMemcpyAsync(mg.host, mg.scal, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    VOX8_HIP(hipStreamSynchronize(st));
+    L.lam_max = 1.1 * mg.host[1];
+    if (!(L.lam_max > 0.0) || !std::isfinite(L.lam_max)) L.lam_max = 2.5;
+  }
+  // coarsest level: assemble and invert
+  if (mg.n_coarse) {
+    const MGLevel& C = mg.lv.back();
+    const int n = mg.n_coarse;
+    hipLaunchKernelGGL(k_mg_coarse_assemble, dim3(nb((int64_t)n * n, 4096)), dim3(256), 0, st, C.g,
+                       (const double*)(mg.lv.size() == 1 ? fK0 : C.fK64), (const uint8_t*)C.fixed, dKe,
+                       C.scale, mg.A);
+    double* rowk = mg.red;
+    double* colk = mg.red + 2048;
+    const unsigned gs = (unsigned)((n + 255) / 256), ge = (unsigned)(((int64_t)n * n + 255) / 256);
+    for (int k = 0; k < n; ++k) {
+      hipLaunchKernelGGL(k_gj_stage, dim3(gs), dim3(256), 0, st, n, k, (const double*)mg.A, rowk, colk);
+      hipLaunchKernelGGL(k_gj_step, dim3(ge), dim3(256), 0, st, n, k, mg.A, (const double*)rowk,
+                         (const double*)colk);
+    }
+    hipLaunchKernelGGL(k_to_f32_n, dim3(nb((int64_t)n * n)), dim3(256), 0, st, (int64_t)n * n,
+                       (const double*)mg.A, mg.Ainv);
+    VOX8_HIP(hipGetLastError());
+  }
+  mg.updated = true;
+  return OK;
+}
+
+struct ChebCoef { double theta, delta, sigma; };
+
+static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st) {
+  const double lmax = L.lam_max, lmin = lmax / alpha;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  float* d = L.d;
+  float* d2 = L.d2;
+  for (int k = 0; k < nu; ++k) {
+    const double rho1 = 1.0 / (2.0 * sigma - rho);
+    Epilogue<float> ep;
+    ep.mode = EPI_CHEB;
+    ep.r = L.r;
+    ep.xs = L.x;
+    ep.dnew = d2;
+    ep.dinv = L.dinv;
+    ep.c1 = (float)(rho1 * rho);
+    ep.c2 = (float)(2.0 * rho1 / delta);
+    ep.ld = L.g.n_dofs;
+    VOX8_TRY(level_apply(L, d, nullptr, ncols, st, ep));
+    std::swap(d, d2);
+    rho = rho1;
+  }
+  return OK;
+}
+
+int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st) {
+  VOX8_CHECK(mg.updated && ncols <= mg.ncols, "multigrid not set up for this density / width");
+  const int nl = (int)mg.lv.size();
+  MGLevel& F = mg.lv[0];
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b);
+  VOX8_HIP(hipGetLastError());
+  const int last_smoothed = mg.n_coarse ? nl - 2 : nl - 1;
+  auto theta_of = [&](const MGLevel& L) { return 0.5 * (L.lam_max + L.lam_max / mg.alpha); };
+  // descending leg
+  for (int l = 0; l <= last_smoothed; ++l) {
+    MGLevel& L = mg.lv[l];
+    const int64_t n = L.g.n_dofs;
+    hipLaunchKernelGGL(k_mg_cheb_init, dim3(nb(n), ncols), dim3(256), 0, st, n, (const float*)L.b,
+                       (const float*)L.dinv, (float)(1.0 / theta_of(L)), L.r, L.d, L.x);
+    VOX8_HIP(hipGetLastError());
+    const int nu = (l == nl - 1) ? 8 * mg.nu : mg.nu;   // no coarse grid below: smooth harder
+    VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st));
+    if (l + 1 < nl) {
+      MGLevel& Cn = mg.lv[l + 1];
+      hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
+                         L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b);
+      VOX8_HIP(hipGetLastError());
+    }
+  }
+  if (mg.n_coarse) {
+    MGLevel& C = mg.lv.back();
+    hipLaunchKernelGGL(k_mg_coarse_apply, dim3((mg.n_coarse + 63) / 64, ncols), dim3(256), 0, st,
+                       mg.n_coarse, (const float*)mg.Ainv, (const float*)C.b, C.x);
+    VOX8_HIP(hipGetLastError());
+  }
+  // ascending leg
+  for (int l = nl - 2; l >= 0; --l) {
+    MGLevel& L = mg.lv[l];
+    const MGLevel& Cn = mg.lv[l + 1];
+    hipLaunchKernelGGL(k_mg_prolong_add, dim3((unsigned)((L.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
+                       Cn.g, L.g, (const float*)Cn.x, (const uint8_t*)L.fixed, L.x);
+    VOX8_HIP(hipGetLastError());
+    Epilogue<float> ep;
+    ep.mode = EPI_RESID;
+    ep.b = L.b;
+    ep.r = L.r;
+    ep.dnew = L.d;
+    ep.dinv = L.dinv;
+    ep.c2 = (float)(1.0 / theta_of(L));
+    ep.ld = L.g.n_dofs;
+    VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
+    VOX8_TRY(smooth(L, mg.nu, mg.alpha, ncols, st));
+  }
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
+                     (const float*)F.x, z);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace vox8

@@ -33,6 +33,10 @@ struct PcgWork {
   int64_t n = 0;
 };
 
+// Preconditioner z = M r for nrhs column-major vectors (ld = n_dofs); r and z may alias.
+template <typename T>
+using Precond = std::function<int(const T* r, T* z, int nrhs, hipStream_t st)>;
+
 struct PcgResult {
   int iters;      // CG iterations (summed over inner solves for the mixed path)
   int info;       // 0 converged, >0 not converged (scipy convention)
@@ -40,14 +44,10 @@ struct PcgResult {
   int outer;      // refinement steps (mixed path), 1 for the plain path
 };
 
+// Batched PCG; M == nullptr: Jacobi (A.dinv) fused into the vector kernels.
 int pcg_fp64(const PcgOps<double>& A, const double* F, int64_t ldf, double* X, bool x0_given,
              int nrhs, double rtol, int maxiter, PcgWork<double>& w, hipStream_t st,
-             PcgResult* res);
-int pcg_mixed(const PcgOps<double>& A64, const PcgOps<float>& A32, const double* F, int64_t ldf,
-              double* X, bool x0_given, int nrhs, double rtol, int maxiter, double inner_rtol,
-              PcgWork<double>& w64, PcgWork<float>& w32, hipStream_t st, PcgResult* res);
-int scale_columns_to_float(const double* a, int64_t lda, double* b, const double* host_scale,
-                           int nrhs, int64_t n, hipStream_t st);
+             PcgResult* res, const Precond<double>* M = nullptr);
 int convert_to_float(const double* a, float* b, int64_t n, hipStream_t st);
 
 // ---- LOBPCG ----
@@ -75,6 +75,7 @@ struct LobpcgWork {
 int lobpcg_fp64(const PcgOps<double>& A, const std::function<int(const double*, double*, double*,
                                                                  int, hipStream_t)>& applyKM,
                 int k, double tol, int maxiter, const double* X0, double* Xout, double* lam_out,
-                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed);
+                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
+                const Precond<double>* M = nullptr);
 
 }  // namespace vox8

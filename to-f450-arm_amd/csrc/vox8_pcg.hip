@@ -68,6 +68,17 @@ k_pcg_init(int64_t n_dofs, const uint8_t* __restrict__ fixed, const double* __re
   }
 }
 
+// scipy cg returns b (= 0) when ||b|| = 0, whatever x0 was.
+template <typename T>
+__global__ void k_pcg_zero_null(int64_t n_dofs, T* __restrict__ x, int64_t ldx,
+                                const PcgScalars* __restrict__ scal) {
+  const int col = blockIdx.y;
+  if (scal[col].bnorm != 0.0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dofs;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[col * ldx + i] = T(0);
+}
+
 // Start of an iteration: stop test, then p = D^-1 r + beta p.
 template <typename T>
 __global__ void k_pcg_dir(int64_t n_dofs, const T* __restrict__ dinv, const T* __restrict__ r,
@@ -89,7 +100,7 @@ __global__ void k_pcg_dir(int64_t n_dofs, const T* __restrict__ dinv, const T* _
 }
 
 // End of an iteration: alpha = rho/(p.q); x += alpha p; r -= alpha q; new rho, ||r||^2.
-template <typename T>
+template <typename T, bool RZ>
 __global__ void __launch_bounds__(256)
 k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
            T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ q, int64_t ld,
@@ -108,17 +119,63 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
     const T rn = (T)((double)r[j] - alpha * (double)q[j]);
     r[j] = rn;
     rr += (double)rn * (double)rn;
-    rz += (double)rn * (double)dinv[i] * (double)rn;
+    if (RZ) rz += (double)rn * (double)dinv[i] * (double)rn;
   }
-  double v[2] = {block_sum(rr, sh), block_sum(rz, sh)}, tot[2];
+  double v[2] = {block_sum(rr, sh), RZ ? block_sum(rz, sh) : 0.0}, tot[2];
   if (grid_reduce_last<2>(v, partials + (int64_t)col * gridDim.x * 2, counters + col, gridDim.x,
                           blockIdx.x, tot, sh)) {
     if (threadIdx.x == 0) {
       scal[col].rr = tot[0];
-      scal[col].rz_prev = s.rz;
-      scal[col].rz = tot[1];
+      if (RZ) {
+        scal[col].rz_prev = s.rz;
+        scal[col].rz = tot[1];
+      }
       scal[col].iters = s.iters + 1;
     }
+  }
+}
+
+// Preconditioned variant (explicit z = M r): stop test, then rho = r.z.
+__global__ void k_pcg_check(PcgScalars* __restrict__ scal, int maxiter) {
+  const int col = blockIdx.y;
+  PcgScalars& s = scal[col];
+  if (!s.done && s.iters < maxiter && s.rr < s.atol2) s.done = 1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pcg_rz(int64_t n_dofs, const T* __restrict__ r, const T* __restrict__ z, int64_t ld,
+         PcgScalars* __restrict__ scal, int maxiter, double* __restrict__ partials,
+         unsigned int* __restrict__ counters) {
+  __shared__ double sh[8];
+  const int col = blockIdx.y;
+  const PcgScalars s = scal[col];
+  if (s.done || s.iters >= maxiter) return;
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dofs;
+       i += (int64_t)gridDim.x * blockDim.x)
+    rz += (double)r[col * ld + i] * (double)z[col * ld + i];
+  double v = block_sum(rz, sh), tot;
+  if (grid_reduce_last<1>(&v, partials + (int64_t)col * gridDim.x, counters + col, gridDim.x,
+                          blockIdx.x, &tot, sh)) {
+    if (threadIdx.x == 0) {
+      scal[col].rz_prev = s.rz;
+      scal[col].rz = tot;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_pcg_dir_z(int64_t n_dofs, const T* __restrict__ z, T* __restrict__ p, int64_t ld,
+                            const PcgScalars* __restrict__ scal, int maxiter) {
+  const int col = blockIdx.y;
+  const PcgScalars s = scal[col];
+  if (s.done || s.iters >= maxiter) return;
+  const double beta = s.iters > 0 ? s.rz / s.rz_prev : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dofs;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = col * ld + i;
+    p[j] = (T)(s.iters > 0 ? (double)z[j] + beta * (double)p[j] : (double)z[j]);
   }
 }
 
@@ -149,7 +206,7 @@ int pcg_reduce_capacity_blocks() { return 1024; }
 template <typename T>
 static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const T* x0_or_null,
                   bool x0_given, int nrhs, double rtol, int maxiter, PcgWork<T>& w,
-                  hipStream_t st, int chunk0) {
+                  hipStream_t st, int chunk0, const Precond<T>* M) {
   const Grid& g = *A.g;
   const int64_t n = g.n_dofs;
   const unsigned nb = vec_blocks(n);
@@ -162,6 +219,7 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
   hipLaunchKernelGGL((k_pcg_init<T>), grid, blk, 0, st, n, A.fixed, F, ldf,
                      x0_given ? (const T*)w.q : (const T*)nullptr, n, A.dinv, w.r, n, rtol,
                      w.scal, w.partials, w.counters);
+  hipLaunchKernelGGL((k_pcg_zero_null<T>), grid, blk, 0, st, n, x, n, (const PcgScalars*)w.scal);
   VOX8_HIP(hipGetLastError());
   DotFuse fuse;
   fuse.scal = w.scal;
@@ -171,10 +229,22 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
   while (launched < maxiter) {
     const int todo = chunk < maxiter - launched ? chunk : maxiter - launched;
     for (int it = 0; it < todo; ++it) {
-      hipLaunchKernelGGL((k_pcg_dir<T>), grid, blk, 0, st, n, A.dinv, w.r, w.p, n, w.scal, maxiter);
-      VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
-      hipLaunchKernelGGL((k_pcg_step<T>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
-                         w.scal, maxiter, w.partials, w.counters);
+      if (!M) {
+        hipLaunchKernelGGL((k_pcg_dir<T>), grid, blk, 0, st, n, A.dinv, w.r, w.p, n, w.scal, maxiter);
+        VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
+        hipLaunchKernelGGL((k_pcg_step<T, true>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
+                           w.scal, maxiter, w.partials, w.counters);
+      } else {
+        hipLaunchKernelGGL(k_pcg_check, dim3(1, nrhs), dim3(1), 0, st, w.scal, maxiter);
+        VOX8_TRY((*M)(w.r, w.x, nrhs, st));
+        hipLaunchKernelGGL((k_pcg_rz<T>), grid, blk, 0, st, n, (const T*)w.r, (const T*)w.x, n,
+                           w.scal, maxiter, w.partials, w.counters);
+        hipLaunchKernelGGL((k_pcg_dir_z<T>), grid, blk, 0, st, n, (const T*)w.x, w.p, n,
+                           (const PcgScalars*)w.scal, maxiter);
+        VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
+        hipLaunchKernelGGL((k_pcg_step<T, false>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
+                           w.scal, maxiter, w.partials, w.counters);
+      }
     }
     VOX8_HIP(hipGetLastError());
     launched += todo;
@@ -193,8 +263,8 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
 
 int pcg_fp64(const PcgOps<double>& A, const double* F, int64_t ldf, double* X, bool x0_given,
              int nrhs, double rtol, int maxiter, PcgWork<double>& w, hipStream_t st,
-             PcgResult* res) {
-  VOX8_TRY(run_cg<double>(A, F, ldf, X, X, x0_given, nrhs, rtol, maxiter, w, st, 8));
+             PcgResult* res, const Precond<double>* M) {
+  VOX8_TRY(run_cg<double>(A, F, ldf, X, X, x0_given, nrhs, rtol, maxiter, w, st, M ? 2 : 8, M));
   for (int j = 0; j < nrhs; ++j) {
     const PcgScalars& s = w.host_scal[j];
     res[j].iters = s.iters;
@@ -202,81 +272,6 @@ int pcg_fp64(const PcgOps<double>& A, const double* F, int64_t ldf, double* X, b
     res[j].relres = s.bnorm > 0 ? sqrt(s.rr) / s.bnorm : 0.0;
     res[j].outer = 1;
   }
-  return OK;
-}
-
-// Mixed precision: FP64 outer residual r = F - K64 x, FP32 inner PCG on the
-// correction, x += d, until ||r|| < rtol ||F_f|| in FP64 (classical iterative
-// refinement; the converged answer is the FP64 one).
-int pcg_mixed(const PcgOps<double>& A64, const PcgOps<float>& A32, const double* F, int64_t ldf,
-              double* X, bool x0_given, int nrhs, double rtol, int maxiter, double inner_rtol,
-              PcgWork<double>& w64, PcgWork<float>& w32, hipStream_t st, PcgResult* res) {
-  const Grid& g = *A64.g;
-  const int64_t n = g.n_dofs;
-  const unsigned nb = vec_blocks(n);
-  const dim3 grid(nb, (unsigned)nrhs), blk(256);
-  hipLaunchKernelGGL((k_mask_copy<double, double>), grid, blk, 0, st, n, A64.fixed,
-                     x0_given ? (const double*)X : (const double*)nullptr, n, X, n);
-  VOX8_HIP(hipGetLastError());
-  for (int j = 0; j < nrhs; ++j) { res[j].iters = 0; res[j].outer = 0; res[j].info = maxiter; }
-  std::vector<int> done(nrhs, 0);
-  int total_inner = 0;
-  for (int outer = 0; outer < 64; ++outer) {
-    // r64 = F - K64 x  (and FP64 norms)
-    VOX8_TRY(A64.apply(X, n, w64.q, n, nrhs, st, DotFuse()));
-    hipLaunchKernelGGL((k_pcg_init<double>), grid, blk, 0, st, n, A64.fixed, F, ldf,
-                       (const double*)w64.q, n, A64.dinv, w64.r, n, rtol, w64.scal,
-                       w64.partials, w64.counters);
-    VOX8_HIP(hipGetLastError());
-    VOX8_HIP(hipMemcpyAsync(w64.host_scal, w64.scal, sizeof(PcgScalars) * nrhs,
-                            hipMemcpyDeviceToHost, st));
-    VOX8_HIP(hipStreamSynchronize(st));
-    bool all = true;
-    for (int j = 0; j < nrhs; ++j) {
-      const PcgScalars& s = w64.host_scal[j];
-      res[j].relres = s.bnorm > 0 ? sqrt(s.rr) / s.bnorm : 0.0;
-      if (s.bnorm == 0.0 || s.rr < s.atol2) {
-        if (!done[j]) { done[j] = 1; res[j].info = 0; res[j].outer = outer; }
-      } else {
-        all = false;
-      }
-    }
-    if (all || total_inner >= maxiter) break;
-    // inner: solve K d = r in FP32 (columns already converged have zero weight:
-    // their rhs is replaced by zeros so the inner CG finishes immediately)
-    for (int j = 0; j < nrhs; ++j) w64.host_scale[j] = done[j] ? 0.0 : 1.0;
-    VOX8_TRY(scale_columns_to_float(w64.r, n, w32.rhs64, w64.host_scale, nrhs, n, st));
-    VOX8_TRY(run_cg<float>(A32, w32.rhs64, n, w32.x, nullptr, false, nrhs, inner_rtol,
-                           maxiter - total_inner, w32, st, 16));
-    int mx = 0;
-    for (int j = 0; j < nrhs; ++j) {
-      if (!done[j]) res[j].iters += w32.host_scal[j].iters;
-      mx = w32.host_scal[j].iters > mx ? w32.host_scal[j].iters : mx;
-    }
-    total_inner += mx > 0 ? mx : 1;
-    hipLaunchKernelGGL((k_add_correction<float>), grid, blk, 0, st, n, (const float*)w32.x, n, X, n);
-    VOX8_HIP(hipGetLastError());
-  }
-  return OK;
-}
-
-__global__ void k_scale_cols(int64_t n, const double* __restrict__ a, int64_t lda,
-                             double* __restrict__ b, int64_t ldb, const double* __restrict__ s) {
-  const int col = blockIdx.y;
-  const double sc = s[col];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    b[col * ldb + i] = sc * a[col * lda + i];
-}
-
-int scale_columns_to_float(const double* a, int64_t lda, double* b, const double* host_scale,
-                           int nrhs, int64_t n, hipStream_t st) {
-  // host_scale lives in pinned memory; copy to the device slot after it
-  double* dscale = b + (int64_t)nrhs * n;
-  VOX8_HIP(hipMemcpyAsync(dscale, host_scale, sizeof(double) * nrhs, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_scale_cols, dim3(vec_blocks(n), nrhs), dim3(256), 0, st, n, a, lda, b, n,
-                     (const double*)dscale);
-  VOX8_HIP(hipGetLastError());
   return OK;
 }
 

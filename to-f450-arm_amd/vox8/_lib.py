@@ -28,6 +28,7 @@ SIGNATURES = {
     "vox8_destroy": [_vp],
     "vox8_set_stream": [_vp, _vp],
     "vox8_set_element_matrices": [_vp, _pf64, _pf64],
+    "vox8_set_stencil": [_vp, _i32, _pi32],
     "vox8_set_topology": [_vp, _vp, _vp],
     "vox8_set_tip_mass": [_vp, C.POINTER(C.c_int64), _i32, _f64],
     "vox8_set_density": [_vp, _vp, _f64, _f64, _f64, _f64],
@@ -36,6 +37,9 @@ SIGNATURES = {
     "vox8_pcg": [_vp, _vp, _vp, _i32, _i64, _i32, _f64, _i32, _i32, _pi32, _pi32, _pf64],
     "vox8_lobpcg": [_vp, _i32, _f64, _i32, _i32, _vp, _vp, _pf64, _pi32, _pi32, _pf64,
                     C.c_uint64],
+    "vox8_precond_apply": [_vp, _vp, _vp, _i32],
+    "vox8_mg_info": [_vp, _pi32, _pi32, _pf64, _i32],
+    "vox8_set_mg_options": [_vp, _i32, _f64, _i32],
     "vox8_sensitivities": [_vp, _vp, _i32, _i64, _pf64, _vp, _f64, _vp, _f64, _f64, _f64,
                            _f64, _vp, _vp, _vp, _pf64, _pf64],
     "vox8_filter_setup": [_vp, _f64, _vp],

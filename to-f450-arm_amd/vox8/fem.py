@@ -220,27 +220,32 @@ class FEM3D:
         return g if self._full_box else g[self._active_idx_dev]
 
     def _set_density(self, rho_active):
-        """Upload densities and form the element factors, skipping the work when
-        the same (unmodified) array was set last.  The last array is held by
-        reference so its identity cannot be reused by another object."""
+        """Upload densities and form the element factors on the device.
+
+        Device tensors are always re-read (kernels may have written them in
+        place without torch noticing); a NumPy array equal to the last one set
+        is skipped, which saves the host-to-device copy.
+        """
         params = (self.SIMP_PENALTY, self.E_MIN_FRAC, self.MASS_THRESHOLD, self.MASS_PENALTY_Q)
-        if isinstance(rho_active, torch.Tensor):
-            key = (rho_active._version, params)
-        else:
+        if not isinstance(rho_active, torch.Tensor):
             rho_active = np.ascontiguousarray(rho_active, dtype=np.float64)
-            key = (hash(rho_active.tobytes()), params)
-        last = self._density_key
-        if last is not None and last[0] is rho_active and last[1] == key:
-            return self._rho_grid
-        if (last is not None and not isinstance(rho_active, torch.Tensor)
-                and isinstance(last[0], np.ndarray) and last[1] == key
-                and np.array_equal(last[0], rho_active)):
-            return self._rho_grid
+            last = self._density_key
+            if (last is not None and last[1] == params and last[0].shape == rho_active.shape
+                    and np.array_equal(last[0], rho_active)):
+                return self._rho_grid
         g = self.grid_from_active(rho_active)
         self._rho_grid = g
         self._call("vox8_set_density", ptr(g), *params)
-        self._density_key = (rho_active if isinstance(rho_active, torch.Tensor) else rho_active.copy(), key)
+        self._density_key = ((rho_active.copy(), params)
+                             if isinstance(rho_active, np.ndarray) else None)
         return g
+
+    def set_stencil(self, kind: str = "auto") -> int:
+        """Select the stencil ("auto" or "dense"); returns the kernel pattern in
+        use (-1 dense 24x24, 0 orthotropic Hadamard blocks, 1 general blocks)."""
+        pat = C.c_int()
+        self._call("vox8_set_stencil", {"auto": 0, "dense": 1}[kind], C.byref(pat))
+        return pat.value
 
     def _set_tip(self, tip_mass_kg, tip_nodes):
         nodes = np.ascontiguousarray(np.asarray(tip_nodes, dtype=np.int64))
