@@ -133,6 +133,11 @@ int vox8_sensitivities(vox8_ctx* ctx, const double* d_U, int nrhs, int64_t ld,
  * taps and the masked denominator; filter applies it (d_in, d_out n_elem). */
 int vox8_filter_setup(vox8_ctx* ctx, double r_min_vox, const uint8_t* d_mask);
 int vox8_filter(vox8_ctx* ctx, const double* d_in, double* d_out);
+/* Transpose of vox8_filter: d_out = H^T (d_in / den) on the mask, the exact
+ * chain rule of the density filter (the reference filters sensitivities with
+ * the forward filter instead, src/topopt.py:162; this entry backs the autograd
+ * path and has no reference counterpart). */
+int vox8_filter_adjoint(vox8_ctx* ctx, const double* d_in, double* d_out);
 
 /* OC update with bisection on the multiplier (src/topopt.py:190-214) over
  * the elements with d_design[e] != 0; other entries of d_xnew copy d_x. */

@@ -4,7 +4,8 @@ Same hierarchy and operators as to-f450-arm_amd/csrc/vox8_mg.hip, written
 with assembled sparse matrices so the preconditioner can be checked on the
 CPU (symmetry, positivity, contraction) and compared with the device kernel:
 levels coarsened 2:1 (ceil), factor = mean of 8 children, element matrix
-2^l Ke, trilinear P (masked by fixed DOFs on both sides), R = P^T, Chebyshev
+2^l Ke, trilinear P (masked by fixed DOFs on both sides; a coarse DOF
+is fixed when any fine node of its 3x3x3 interpolation stencil has it fixed), R = P^T, Chebyshev
 smoothing of D^-1 K on [lam_max/alpha, lam_max], dense coarsest solve.
 """
 from __future__ import annotations
@@ -92,8 +93,9 @@ class VCycle:
             for K_ in range(cz + 1):
                 for J in range(cy + 1):
                     for I in range(cx + 1):
-                        if 2 * I <= nx and 2 * J <= ny and 2 * K_ <= nz:
-                            fc_fix[K_, J, I] = fx3[2 * K_, 2 * J, 2 * I]
+                        blk = fx3[max(2 * K_ - 1, 0):2 * K_ + 2, max(2 * J - 1, 0):2 * J + 2,
+                                  max(2 * I - 1, 0):2 * I + 2]
+                        fc_fix[K_, J, I] = np.bitwise_or.reduce(blk.ravel()) if blk.size else 0
             fc_fix[~used] = 7
             fixed_c = fc_fix.ravel()
             _, free_c = assemble(cx, cy, cz, fc, Ke, fixed_c)

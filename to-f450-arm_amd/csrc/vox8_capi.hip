@@ -587,7 +587,7 @@ int vox8_filter_setup(vox8_ctx* c, double r_min_vox, const uint8_t* d_mask) {
   VOX8_HIP(hipMemcpyAsync(c->d_tw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice, c->st));
   VOX8_HIP(hipMemcpyAsync(c->d_fmask, d_mask, c->g.n_elem, hipMemcpyDeviceToDevice, c->st));
   VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask,
-                         nullptr, nullptr, c->d_den, c->st));
+                         nullptr, nullptr, c->d_den, FILTER_DEN, c->st));
   VOX8_HIP(hipStreamSynchronize(c->st));
   c->have_filter = true;
   return OK;
@@ -600,7 +600,17 @@ int vox8_filter(vox8_ctx* c, const double* d_in, double* d_out) {
   VOX8_CHECK(d_in != d_out, "filter cannot run in place");
   if (!c->have_filter) { set_error("filter not set up (vox8_filter_setup)"); return ERR_STATE; }
   return launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask, c->d_den,
-                       d_in, d_out, c->st);
+                       d_in, d_out, FILTER_FWD, c->st);
+  API_END
+}
+
+int vox8_filter_adjoint(vox8_ctx* c, const double* d_in, double* d_out) {
+  API_BEGIN
+  VOX8_CHECK(c && d_in && d_out, "null argument");
+  VOX8_CHECK(d_in != d_out, "filter cannot run in place");
+  if (!c->have_filter) { set_error("filter not set up (vox8_filter_setup)"); return ERR_STATE; }
+  return launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask, c->d_den,
+                       d_in, d_out, FILTER_ADJ, c->st);
   API_END
 }
 

@@ -9,7 +9,8 @@
 //     (exact for any constant material), factor = mean of the 8 children
 //     (rediscretisation);
 //   * transfer: trilinear prolongation P, restriction R = P^T, both masked by
-//     the fixed DOFs of their level;
+//     the fixed DOFs of their level (a coarse DOF is fixed when a fine node of
+//     its interpolation stencil has it fixed);
 //   * smoother: Chebyshev polynomial of D^-1 K on [lam_max/alpha, lam_max]
 //     (three-term recurrence, Saad, Iterative Methods, sec. 12.1), nu steps
 //     before and after; lam_max from power iteration, +10 % margin;
@@ -170,12 +171,26 @@ __global__ void k_mg_coarse_assemble(Grid g, const double* __restrict__ f, const
 //   a_ik = -a_ik p (i != k); a_kk = p.
 // The pivot row is read in its pre-step state from `rowk`, the pivot column
 // from `colk` (both copied by k_gj_stage), so the update has no read/write hazard.
+// The pivot of step k is the Schur-complement diagonal, positive for an SPD
+// matrix; a pivot below 1e-10 of the original diagonal marks the coarse matrix
+// singular (flag = 1; step 0 also clears the flag).
 __global__ void k_gj_stage(int n, int k, const double* __restrict__ A, double* __restrict__ rowk,
-                           double* __restrict__ colk) {
+                           double* __restrict__ colk, const double* __restrict__ diag0,
+                           double* __restrict__ flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   rowk[i] = A[(int64_t)k * n + i];
   colk[i] = A[(int64_t)i * n + k];
+  if (i == k) {
+    const bool bad = !(A[(int64_t)k * n + k] > 1e-10 * diag0[k]);
+    if (k == 0) *flag = bad ? 1.0 : 0.0;
+    else if (bad) *flag = 1.0;
+  }
+}
+
+__global__ void k_gj_diag(int n, const double* __restrict__ A, double* __restrict__ diag0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) diag0[i] = fabs(A[(int64_t)i * n + i]);
 }
 
 __global__ void k_gj_step(int n, int k, double* __restrict__ A, const double* __restrict__ rowk,
@@ -335,9 +350,17 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
       for (int J = 0; J < gc.NY; ++J)
         for (int I = 0; I < gc.NX; ++I) {
           const int64_t c = I + (int64_t)gc.NX * (J + (int64_t)gc.NY * K);
+          // a coarse DOF is fixed when any fine node it interpolates to (the
+          // 3x3x3 fine neighbourhood of its position) has that DOF fixed, so a
+          // clamp on odd-indexed nodes survives coarsening
           uint8_t f = 0;
-          if (2 * I < g.NX && 2 * J < g.NY && 2 * K < g.NZ)
-            f = fix[2 * I + (int64_t)g.NX * (2 * J + (int64_t)g.NY * 2 * K)];
+          for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+              for (int dx = -1; dx <= 1; ++dx) {
+                const int x = 2 * I + dx, y = 2 * J + dy, z = 2 * K + dz;
+                if (x < 0 || y < 0 || z < 0 || x >= g.NX || y >= g.NY || z >= g.NZ) continue;
+                f |= fix[x + (int64_t)g.NX * (y + (int64_t)g.NY * z)];
+              }
           if (!used[c]) f = 7;
           fixc[c] = f;
         }
@@ -353,7 +376,7 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * mg.n_coarse * mg.n_coarse)); mg.A = (double*)p;
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * mg.n_coarse * mg.n_coarse)); mg.Ainv = (float*)p;
   }
-  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 4096)); mg.red = (double*)p;
+  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 6144)); mg.red = (double*)p;
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(unsigned int) * 4)); mg.counter = (unsigned int*)p;
   VOX8_HIP(hipMemset(mg.counter, 0, sizeof(unsigned int) * 4));
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 8)); mg.scal = (double*)p;
@@ -432,15 +455,25 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
                        C.scale, mg.A);
     double* rowk = mg.red;
     double* colk = mg.red + 2048;
+    double* diag0 = mg.red + 4096;
     const unsigned gs = (unsigned)((n + 255) / 256), ge = (unsigned)(((int64_t)n * n + 255) / 256);
+    hipLaunchKernelGGL(k_gj_diag, dim3(gs), dim3(256), 0, st, n, (const double*)mg.A, diag0);
     for (int k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_gj_stage, dim3(gs), dim3(256), 0, st, n, k, (const double*)mg.A, rowk, colk);
+      hipLaunchKernelGGL(k_gj_stage, dim3(gs), dim3(256), 0, st, n, k, (const double*)mg.A, rowk, colk,
+                         (const double*)diag0, mg.scal + 7);
       hipLaunchKernelGGL(k_gj_step, dim3(ge), dim3(256), 0, st, n, k, mg.A, (const double*)rowk,
                          (const double*)colk);
     }
     hipLaunchKernelGGL(k_to_f32_n, dim3(nb((int64_t)n * n)), dim3(256), 0, st, (int64_t)n * n,
                        (const double*)mg.A, mg.Ainv);
     VOX8_HIP(hipGetLastError());
+    VOX8_HIP(hipMemcpyAsync(mg.host + 7, mg.scal + 7, sizeof(double), hipMemcpyDeviceToHost, st));
+    VOX8_HIP(hipStreamSynchronize(st));
+    if (mg.host[7] != 0.0) {
+      set_error("multigrid: the coarsest-level stiffness matrix is singular (a part of the "
+                "structure is not held by fixed DOFs on the coarse grid)");
+      return ERR_NUMERIC;
+    }
   }
   mg.updated = true;
   return OK;

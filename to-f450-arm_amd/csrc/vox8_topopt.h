@@ -3,9 +3,10 @@
 #include "vox8_kernels.h"
 
 namespace vox8 {
+enum FilterMode { FILTER_FWD = 0, FILTER_ADJ = 1, FILTER_DEN = 2 };
 int launch_filter(int nx, int ny, int nz, int ntap, const int* toff, const double* tw,
                   const uint8_t* mask, const double* den, const double* in, double* out,
-                  hipStream_t st);
+                  int mode, hipStream_t st);
 int launch_quadforms(const Grid& g, const double* fK, const double* fM, const uint8_t* active,
                      const double* Ke, const double* Me, const double* U, int64_t ldu, int nrhs,
                      const double* wts, const double* phi, double* ce, double* kphi,

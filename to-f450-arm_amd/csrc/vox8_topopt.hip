@@ -47,17 +47,20 @@ __device__ bool grid_reduce_last_dyn(const double* vals /*smem, nv*/, int nv,
 }
 
 // ---- masked cone filter ------------------------------------------------------
-// out[e] = mask[e] ? sum_t w_t in[e+t] mask[e+t] / den[e] : 0.  With in == null
-// the kernel writes the denominator max(sum_t w_t mask[e+t], 1e-12) instead.
+// mode FILTER_FWD:  out[e] = mask[e] ? sum_t w_t in[e+t] mask[e+t] / den[e] : 0
+// mode FILTER_ADJ:  out[e] = mask[e] ? sum_t w_t in[e+t] mask[e+t] / den[e+t] : 0
+//                   (the transpose of FILTER_FWD: the tap set is symmetric)
+// mode FILTER_DEN:  out[e] = max(sum_t w_t mask[e+t], 1e-12)  (the denominator)
 __global__ void __launch_bounds__(256)
 k_filter(int nx, int ny, int nz, int ntap, const int* __restrict__ toff,
          const double* __restrict__ tw, const uint8_t* __restrict__ mask,
-         const double* __restrict__ den, const double* __restrict__ in, double* __restrict__ out) {
+         const double* __restrict__ den, const double* __restrict__ in, double* __restrict__ out,
+         int mode) {
   const int64_t n = (int64_t)nx * ny * nz;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((int64_t)nx * ny));
-    if (in && !mask[e]) { out[e] = 0.0; continue; }
+    if (mode != FILTER_DEN && !mask[e]) { out[e] = 0.0; continue; }
     double s = 0.0;
     for (int t = 0; t < ntap; ++t) {
       const int dx = toff[3 * t], dy = toff[3 * t + 1], dz = toff[3 * t + 2];
@@ -65,17 +68,18 @@ k_filter(int nx, int ny, int nz, int ntap, const int* __restrict__ toff,
       if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
       const int64_t f = x + (int64_t)nx * (y + (int64_t)ny * z);
       if (!mask[f]) continue;
-      s += tw[t] * (in ? in[f] : 1.0);
+      const double v = mode == FILTER_DEN ? 1.0 : mode == FILTER_ADJ ? in[f] / den[f] : in[f];
+      s += tw[t] * v;
     }
-    out[e] = in ? s / den[e] : fmax(s, 1e-12);
+    out[e] = mode == FILTER_FWD ? s / den[e] : mode == FILTER_ADJ ? s : fmax(s, 1e-12);
   }
 }
 
 int launch_filter(int nx, int ny, int nz, int ntap, const int* toff, const double* tw,
                   const uint8_t* mask, const double* den, const double* in, double* out,
-                  hipStream_t st) {
+                  int mode, hipStream_t st) {
   hipLaunchKernelGGL(k_filter, dim3(nblk((int64_t)nx * ny * nz, 4096)), dim3(256), 0, st, nx, ny, nz,
-                     ntap, toff, tw, mask, den, in, out);
+                     ntap, toff, tw, mask, den, in, out, mode);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

@@ -44,6 +44,7 @@ SIGNATURES = {
                            _f64, _vp, _vp, _vp, _pf64, _pf64],
     "vox8_filter_setup": [_vp, _f64, _vp],
     "vox8_filter": [_vp, _vp, _vp],
+    "vox8_filter_adjoint": [_vp, _vp, _vp],
     "vox8_oc_update": [_vp, _vp, _vp, _vp, _f64, _f64, _f64, _vp, _pf64],
     "vox8_stats": [_vp, _i64, _vp, _vp, _vp, _pf64],
     "vox8_block_dots": [_vp, _i64, _i32, _vp, _i64, _vp, _pf64],
