@@ -49,3 +49,47 @@ def rel(a, b):
     a = np.asarray(a, dtype=float)
     b = np.asarray(b, dtype=float)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+class TipBox(MaskedBox):
+    """MaskedBox with the load/tip-mass interface the SIMP loops read
+    (load_dofs_for_force, motor_mount_top_nodes): forces are spread evenly
+    over the x = nx face nodes, the tip mass over that face's top edge."""
+
+    def _tip_face(self):
+        iy, iz = np.meshgrid(np.arange(self.ny + 1), np.arange(self.nz + 1), indexing="ij")
+        return np.sort(self.node_index(self.nx, iy, iz).ravel()).astype(np.int64)
+
+    def load_dofs_for_force(self, F_xyz):
+        nodes = self._tip_face()
+        dofs, mags = [], []
+        for c, f in enumerate(F_xyz):
+            if abs(f) > 1e-12:
+                dofs.append(3 * nodes + c)
+                mags.append(np.full(len(nodes), f / len(nodes)))
+        if not dofs:
+            return np.array([], dtype=np.int64), np.array([])
+        return np.concatenate(dofs), np.concatenate(mags)
+
+    def load_dofs_for_moment_Mz(self, Mz):
+        return np.array([], dtype=np.int64), np.array([])
+
+    def motor_mount_top_nodes(self):
+        return np.sort(self.node_index(self.nx, np.arange(self.ny + 1), self.nz)).astype(np.int64)
+
+
+# inner-solve tolerances of the SIMP fixture (the loop itself does not forward
+# them: src/fem.py:558-561, src/topopt.py:315-321)
+TIPBOX_SOLVE_RTOL = 1e-12
+TIPBOX_MODAL_TOL = 1e-5
+TIPBOX_MODAL_MAXITER = 4000
+
+
+def tipbox_problem():
+    """The SIMP fixture problem (tests/golden/simp_tipbox.npz)."""
+    geom = TipBox(12, 4, 4, h=1e-3, hole_frac=0.08, seed=11)
+    loads = [{"F": (0.0, 0.0, -1.0)}, {"F": (0.0, 0.6, 0.0)}]
+    opts = dict(vol_frac=0.3, r_min_mm=1.5, move=0.2, max_iter=8, weights=[0.7, 0.3],
+                min_freq_Hz=300.0, tip_mass_kg=2e-3, n_modes_track=3)
+    mat = dict(E=1.7e9, nu=0.39, rho=930.0)
+    return geom, loads, opts, mat

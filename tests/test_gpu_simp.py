@@ -1,0 +1,36 @@
+"""GPU SIMP loop (vox8.topopt.SIMPOptimizer) against the reference's own run.
+
+Same fixture as tests/test_oracle_simp.py (tests/golden/simp_tipbox.npz,
+written by the reference's SIMPOptimizer with converged inner solves): eight
+iterations with two weighted load cases, tip mass, an active frequency
+constraint, MAC mode tracking, the mu ramp and the OC bisection, all on the
+device.  Bar: compliance and frequencies within 1e-6 relative; the rest of
+the history and the final densities within 1e-6.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from helpers import (TIPBOX_MODAL_MAXITER, TIPBOX_MODAL_TOL, TIPBOX_SOLVE_RTOL, rel,
+                     tipbox_problem)
+
+pytestmark = pytest.mark.gpu
+
+HIST = ("compliance", "mass", "change", "vol_frac", "omega_1_Hz", "mu_freq", "freq_violation")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_simp_loop_matches_reference_run(gpu, prec):
+    from vox8.fem import FEM3D, Material
+    from vox8.topopt import SIMPOptimizer
+    gold = np.load(GOLDEN / "simp_tipbox.npz")
+    geom, loads, opts, mat = tipbox_problem()
+    fem = FEM3D(geom, Material(**mat), precision=prec)
+    res = SIMPOptimizer(geom, fem, loads, solve_rtol=TIPBOX_SOLVE_RTOL, modal_tol=TIPBOX_MODAL_TOL,
+                        modal_maxiter=TIPBOX_MODAL_MAXITER, **opts).run(verbose=False)
+    h = res["history"]
+    errs = {k: rel(np.asarray(h[k], dtype=float), gold[f"hist_{k}"]) for k in HIST}
+    errs["rho"] = rel(res["rho_active"], gold["rho_active"])
+    errs["u"] = rel(np.stack(res["u_per_lc"]), gold["u_per_lc"])
+    print(prec, {k: f"{v:.1e}" for k, v in errs.items()})
+    assert all(v < 1e-6 for v in errs.values()), errs

@@ -106,19 +106,21 @@ def _general_D():
     return (A @ A.T + 6 * np.eye(6)) * 1e8
 
 
-@pytest.mark.parametrize("kind", ["iso", "ortho", "general"])
+@pytest.mark.parametrize("kind", ["iso", "iso_box", "ortho", "general"])
 def test_fast_stencil_patterns_vs_dense(gpu, kind):
     """The structured kernel (each pattern) against the dense 24x24 kernel and the oracle."""
     from oracle.element import element_K, ortho_D
-    geom = MaskedBox(40, 11, 9, seed=3)            # spans several tiles in x, y and z
+    geom = MaskedBox(70, 11, 9, seed=3)            # spans several tiles in x, y and z
+    if kind == "iso_box":                          # isotropic, non-cubic voxel
+        geom.dy, geom.dz = 1.3e-3, 0.8e-3
     ofem, vfem, rho = _pair(geom)
-    if kind != "iso":
+    if kind in ("ortho", "general"):
         D = ortho_D(2.0e9, 1.2e9, 1.0e9, 0.35, 0.3, 0.33, 0.6e9, 0.4e9, 0.5e9) if kind == "ortho" else _general_D()
         Ke = element_K(0, 0, geom.dx, geom.dy, geom.dz, D=D)
         ofem.Ke_solid, ofem.Ke_flat = Ke, Ke.ravel()
         vfem.Ke_solid = Ke.copy()
     pat = vfem.set_stencil("auto")
-    assert pat == {"iso": 0, "ortho": 0, "general": 1}[kind]
+    assert pat == {"iso": 2, "iso_box": 0, "ortho": 0, "general": 1}[kind]
     X = np.random.default_rng(7).standard_normal((geom.n_dofs, 3))
     yk = vfem.assemble_K(rho) @ X
     ym = vfem.assemble_M(rho) @ X

@@ -179,8 +179,8 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   VOX8_TRY(dalloc(c, &w.AS, (size_t)3 * k * n));
   VOX8_TRY(dalloc(c, &w.BS, (size_t)3 * k * n));
   VOX8_TRY(dalloc(c, &w.G, (size_t)(2 * 36 * 36 + 36 * 12 + 12 + 8)));
-  int64_t chunks = (n + 8191) / 8192;
-  if (chunks > 256) chunks = 256;
+  int64_t chunks = (n + 511) / 512;          // Gram blocks: >= 8 row chunks of 64 each
+  if (chunks > 512) chunks = 512;
   if (chunks < 1) chunks = 1;
   w.gram_chunks = (int)chunks;
   VOX8_TRY(dalloc(c, &w.gram_partials, (size_t)chunks * 2 * 36 * 36));

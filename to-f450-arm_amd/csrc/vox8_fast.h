@@ -4,7 +4,10 @@
 
 namespace vox8 {
 
-enum Pattern : int { PAT_DENSE = -1, PAT_ORTHO = 0, PAT_BLOCK = 1 };
+// PAT_ORTHO: orthotropic non-zero pattern (45 entries); PAT_BLOCK: general
+// blocks; PAT_CUBE: the orthotropic pattern with the equalities of an isotropic
+// material on a cubic voxel (8 distinct stiffness and 4 distinct mass values).
+enum Pattern : int { PAT_DENSE = -1, PAT_ORTHO = 0, PAT_BLOCK = 1, PAT_CUBE = 2 };
 
 // Element operator in the monomial basis, scaled by 1/64 (see vox8_stencil_fast.hip):
 // lin: {xi, eta, zeta} x {x,y,z}, bil: {xi eta, eta zeta, xi zeta} x {x,y,z},
@@ -25,8 +28,16 @@ constexpr int ORTHO_BIL[21][2] = {{0, 0}, {0, 5}, {1, 1}, {1, 8}, {2, 2}, {2, 3}
                                   {3, 2}, {3, 3}, {3, 7}, {4, 4}, {4, 6}, {5, 0}, {5, 5},
                                   {6, 4}, {6, 6}, {7, 2}, {7, 3}, {7, 7}, {8, 1}, {8, 8}};
 
+// Classes of the 21 lin / 21 bil / 3 tri orthotropic entries (ORTHO_LIN,
+// ORTHO_BIL order, then tri diagonal) under cubic isotropy.  The structured
+// kernel reads the class values from lin[0][0], lin[0][4], lin[1][1],
+// bil[0][0], bil[0][5], bil[2][2], bil[2][3], tri[0][0].
+constexpr int CUBE_CLASS[45] = {0, 1, 1, 2, 2, 2, 2, 2, 2, 1, 0, 1, 2, 2, 2, 2, 2, 2, 1, 1, 0,
+                                3, 4, 3, 4, 5, 6, 6, 6, 5, 6, 3, 4, 4, 3, 4, 3, 6, 6, 5, 4, 3,
+                                7, 7, 7};
+
 // Classify Ke/Me (24x24 row-major, reference local node order) and derive the
-// coefficients.  Returns PAT_ORTHO, PAT_BLOCK or PAT_DENSE.
+// coefficients.  Returns PAT_CUBE, PAT_ORTHO, PAT_BLOCK or PAT_DENSE.
 int fast_coefficients(const double* Ke, const double* Me, FastCoef<double>* c64,
                       FastCoef<float>* c32);
 

@@ -23,7 +23,6 @@
 
 namespace vox8 {
 
-constexpr int GT = 6;           // Gram register tile
 constexpr int MMAX = 36;        // largest Rayleigh-Ritz basis (k <= 12)
 
 // ---- counter-based normal random numbers for the default start block --------
@@ -50,138 +49,204 @@ __global__ void k_random_block(int64_t n, int k, uint64_t seed, const uint8_t* _
 }
 
 // ---- Gram matrices GA = S^T AS, GB = S^T BS over the first m columns ---------
-// grid = (chunks, 2 * tiles^2); each thread accumulates a GT x GT tile.
+// Each block stages GR-row chunks of all 3m columns in LDS (every HBM value is
+// read once; the next chunk is prefetched into registers while the current
+// one is reduced) and accumulates 3x3 tiles of the upper-triangular tile pairs
+// of both matrices; a task (matrix, tile pair) is split over nsl row slices.
+// Per-block partials [block][2][m][m]; off-diagonal tiles are mirrored.
+constexpr int GR = 64;
+constexpr int GPF = (MMAX + 3) / 4;        // columns per wave and matrix
+size_t gram_lds_bytes(int m) {
+  const size_t stage = (size_t)3 * m * (GR + 1), red = 256 * 9;
+  return sizeof(double) * (stage > red ? stage : red);
+}
+
 __global__ void __launch_bounds__(256)
 k_gram(int64_t n, int m, const double* __restrict__ S, const double* __restrict__ AS,
        const double* __restrict__ BS, double* __restrict__ partials) {
-  __shared__ double red[4][GT * GT];
-  const int nt = (m + GT - 1) / GT;
-  const int tile = blockIdx.y;
-  const int gsel = tile / (nt * nt);
-  const int ti = (tile % (nt * nt)) / nt, tj = tile % nt;
-  const double* T = gsel ? BS : AS;
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t r0 = chunk * blockIdx.x;
-  const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
-  double acc[GT][GT];
-#pragma unroll
-  for (int a = 0; a < GT; ++a)
-#pragma unroll
-    for (int b = 0; b < GT; ++b) acc[a][b] = 0.0;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    double s[GT], t[GT];
-#pragma unroll
-    for (int a = 0; a < GT; ++a) {
-      const int ci = ti * GT + a, cj = tj * GT + a;
-      s[a] = ci < m ? S[(int64_t)ci * n + r] : 0.0;
-      t[a] = cj < m ? T[(int64_t)cj * n + r] : 0.0;
-    }
-#pragma unroll
-    for (int a = 0; a < GT; ++a)
-#pragma unroll
-      for (int b = 0; b < GT; ++b) acc[a][b] += s[a] * t[b];
+  extern __shared__ double gsh[];            // [3 m][GR + 1], then 256 x 9 for the reduction
+  const int nt = (m + 2) / 3;
+  const int ntp = nt * (nt + 1) / 2;
+  const int ntask = 2 * ntp;
+  const int nsl = ntask >= 256 ? 1 : 256 / ntask;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int task = t / nsl, sl = t - task * nsl;
+  const bool active = task < ntask;
+  int gsel = 0, ti = 0, tj = 0;
+  if (active) {
+    gsel = task / ntp;
+    int p = task - gsel * ntp;
+    while (p >= nt - ti) { p -= nt - ti; ++ti; }
+    tj = ti + p;
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc[3][3];
 #pragma unroll
-  for (int a = 0; a < GT; ++a)
+  for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int b = 0; b < GT; ++b) {
-      const double v = wave_sum(acc[a][b]);
-      if (lane == 0) red[w][a * GT + b] = v;
+    for (int j = 0; j < 3; ++j) acc[i][j] = 0.0;
+  // staging role: wave wv loads columns wv, wv + 4, ... of each matrix, row = lane
+  double pf[3][GPF];
+  auto fetch = [&](int64_t c0) {
+    const int64_t r = c0 + lane;
+#pragma unroll
+    for (int mat = 0; mat < 3; ++mat) {
+      const double* src = mat == 0 ? S : (mat == 1 ? AS : BS);
+#pragma unroll
+      for (int i = 0; i < GPF; ++i) {
+        const int cc = wv + 4 * i;
+        pf[mat][i] = (cc < m && r < n) ? src[(int64_t)cc * n + r] : 0.0;
+      }
     }
+  };
+  const int64_t step = (int64_t)gridDim.x * GR;
+  int64_t c0 = (int64_t)blockIdx.x * GR;
+  if (c0 < n) fetch(c0);
+  for (; c0 < n; c0 += step) {
+#pragma unroll
+    for (int mat = 0; mat < 3; ++mat)
+#pragma unroll
+      for (int i = 0; i < GPF; ++i) {
+        const int cc = wv + 4 * i;
+        if (cc < m) gsh[(mat * m + cc) * (GR + 1) + lane] = pf[mat][i];
+      }
+    __syncthreads();
+    if (c0 + step < n) fetch(c0 + step);
+    if (active) {
+      const int rows = (int)(n - c0 < GR ? n - c0 : GR);
+      const double* A0 = gsh + (3 * ti) * (GR + 1);
+      const double* B0 = gsh + ((1 + gsel) * m + 3 * tj) * (GR + 1);
+      for (int r = sl; r < rows; r += nsl) {
+        double a[3], bv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          a[i] = 3 * ti + i < m ? A0[i * (GR + 1) + r] : 0.0;
+          bv[i] = 3 * tj + i < m ? B0[i * (GR + 1) + r] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) acc[i][j] += a[i] * bv[j];
+      }
+    }
+    __syncthreads();
+  }
+  double* red = gsh;                          // 256 x 9 doubles
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) red[t * 9 + 3 * i + j] = acc[i][j];
   __syncthreads();
-  if (threadIdx.x < GT * GT) {
-    const int a = threadIdx.x / GT, b = threadIdx.x % GT;
-    const double v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-    const int ci = ti * GT + a, cj = tj * GT + b;
-    if (ci < m && cj < m)
-      partials[((int64_t)blockIdx.x * 2 + gsel) * m * m + ci * m + cj] = v;
+  if (active && sl == 0) {
+    double* out = partials + ((int64_t)blockIdx.x * 2 + gsel) * m * m;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double v = 0.0;
+        for (int q = 0; q < nsl; ++q) v += red[(t + q) * 9 + 3 * i + j];
+        const int ci = 3 * ti + i, cj = 3 * tj + j;
+        if (ci < m && cj < m) {
+          out[ci * m + cj] = v;
+          if (ti != tj) out[cj * m + ci] = v;
+        }
+      }
   }
 }
 
-// Sum the chunk partials in a fixed order into G = [GA | GB] (m*m each).
-__global__ void k_gram_finish(int chunks, int m, const double* __restrict__ partials,
-                              double* __restrict__ G) {
+// Sum the block partials in a fixed order into G = [GA | GB] (m*m each): one
+// wavefront per entry.
+__global__ void __launch_bounds__(64)
+k_gram_finish(int chunks, int m, const double* __restrict__ partials, double* __restrict__ G) {
   const int tot = 2 * m * m;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += partials[(int64_t)c * tot + e];
-    G[e] = s;
-  }
+  const int e = blockIdx.x;
+  double s = 0.0;
+  for (int c = threadIdx.x; c < chunks; c += 64) s += partials[(int64_t)c * tot + e];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) G[e] = s;
 }
 
-// ---- one-wavefront dense symmetric eigen-solver (cyclic Jacobi) ------------
+// ---- dense symmetric eigen-solver in one workgroup (parallel cyclic Jacobi) --
 // A (m x m, row stride MMAX, in LDS) is diagonalised in place; V accumulates
 // the rotations.  Parallel ordering: the circle method pairs all indices into
-// disjoint rotations per round (Golub & Van Loan, sec. 8.5.5).  Launched with
-// one 64-wide wavefront, so the barriers are cheap.
-__device__ void jacobi_eig(double* A, double* V, int m) {
+// disjoint rotations per round (Golub & Van Loan, sec. 8.5.5); an odd m gets a
+// dummy index, whose partner is left unrotated that round.  One round is one
+// pass: thread roles are the 2x2 blocks (pair a, pair b) of A, which become
+// R_a^T A_ab R_b, and the (row, pair) segments of V, which become V_j,ab R_b;
+// every entry belongs to exactly one role, so the pass has no hazards.
+constexpr int RR_THREADS = 256;
+
+__device__ void jacobi_eig(double* A, double* V, int m, double eps) {
   __shared__ double rc[MMAX / 2 + 1], rs[MMAX / 2 + 1];
   __shared__ int rp[MMAX / 2 + 1], rq[MMAX / 2 + 1];
+  __shared__ int s_rot;
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int e = tid; e < m * m; e += nth) V[(e / m) * MMAX + e % m] = (e / m == e % m) ? 1.0 : 0.0;
-  const int mm = (m & 1) ? m + 1 : m;      // odd m: one idle "bye" slot
+  const int mm = (m & 1) ? m + 1 : m;
   const int npair = mm / 2;
-  __syncthreads();
+  const int nA = npair * npair, nroles = nA + m * npair;
+  // threshold cyclic Jacobi: a rotation is applied when |a_pq| exceeds
+  // eps sqrt(|a_pp a_qq|); a sweep that applies none ends the iteration
+  // (off-diagonal entries are then at the rounding level of the diagonal)
   for (int sweep = 0; sweep < 40; ++sweep) {
-    double off = 0.0, fro = 0.0;
-    for (int e = tid; e < m * m; e += nth) {
-      const double a = A[(e / m) * MMAX + e % m];
-      fro += a * a;
-      if (e / m != e % m) off += a * a;
-    }
-    off = wave_sum(off);
-    fro = wave_sum(fro);
-    if (off <= 1e-30 * fro || off == 0.0) break;
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
     for (int round = 0; round < mm - 1; ++round) {
       if (tid < npair) {
         const int i = tid;
-        const int a = (i == 0) ? 0 : 1 + (i - 1 + round) % (mm - 1);
-        const int b = 1 + (mm - 2 - i + round) % (mm - 1);
-        int p = a < b ? a : b, q = a < b ? b : a;
-        double c = 1.0, s = 0.0;
+        const int x = (i == 0) ? 0 : 1 + (i - 1 + round) % (mm - 1);
+        const int y = 1 + (mm - 2 - i + round) % (mm - 1);
+        int p = x < y ? x : y, q = x < y ? y : x;
+        double c = 1.0, sn = 0.0;
         if (q < m) {
           const double apq = A[p * MMAX + q];
           const double app = A[p * MMAX + p], aqq = A[q * MMAX + q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > 1e-18 * sqrt(fabs(app) * fabs(aqq))) {
+          if (fabs(apq) > 1e-300 && fabs(apq) > eps * sqrt(fabs(app) * fabs(aqq))) {
             const double tau = (aqq - app) / (2.0 * apq);
             const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
                                         : -1.0 / (-tau + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
+            sn = t * c;
+            s_rot = 1;
           }
         } else {
-          p = q = -1;
+          q = -1;                           // p is paired with the dummy index
         }
-        rc[i] = c; rs[i] = s; rp[i] = p; rq[i] = q;
+        rc[i] = c; rs[i] = sn; rp[i] = p; rq[i] = q;
       }
       __syncthreads();
-      for (int e = tid; e < npair * m; e += nth) {          // rows: A <- J^T A
-        const int i = e / m, j = e % m;
-        const int p = rp[i], q = rq[i];
-        if (p < 0 || rs[i] == 0.0) continue;
-        const double c = rc[i], s = rs[i];
-        const double ap = A[p * MMAX + j], aq = A[q * MMAX + j];
-        A[p * MMAX + j] = c * ap - s * aq;
-        A[q * MMAX + j] = s * ap + c * aq;
-      }
-      __syncthreads();
-      for (int e = tid; e < npair * m; e += nth) {          // columns: A <- A J, V <- V J
-        const int i = e / m, j = e % m;
-        const int p = rp[i], q = rq[i];
-        if (p < 0 || rs[i] == 0.0) continue;
-        const double c = rc[i], s = rs[i];
-        const double ap = A[j * MMAX + p], aq = A[j * MMAX + q];
-        A[j * MMAX + p] = c * ap - s * aq;
-        A[j * MMAX + q] = s * ap + c * aq;
-        const double vp = V[j * MMAX + p], vq = V[j * MMAX + q];
-        V[j * MMAX + p] = c * vp - s * vq;
-        V[j * MMAX + q] = s * vp + c * vq;
+      for (int e = tid; e < nroles; e += nth) {
+        if (e < nA) {                       // A block (a, b)
+          const int ia = e / npair, ib = e - ia * npair;
+          const int p1 = rp[ia], q1 = rq[ia], p2 = rp[ib], q2 = rq[ib];
+          const double c1 = rc[ia], s1 = rs[ia], c2 = rc[ib], s2 = rs[ib];
+          if (s1 == 0.0 && s2 == 0.0) continue;
+          const double a_pp = A[p1 * MMAX + p2];
+          const double a_pq = q2 >= 0 ? A[p1 * MMAX + q2] : 0.0;
+          const double a_qp = q1 >= 0 ? A[q1 * MMAX + p2] : 0.0;
+          const double a_qq = (q1 >= 0 && q2 >= 0) ? A[q1 * MMAX + q2] : 0.0;
+          const double t_pp = c1 * a_pp - s1 * a_qp, t_qp = s1 * a_pp + c1 * a_qp;
+          const double t_pq = c1 * a_pq - s1 * a_qq, t_qq = s1 * a_pq + c1 * a_qq;
+          A[p1 * MMAX + p2] = c2 * t_pp - s2 * t_pq;
+          if (q2 >= 0) A[p1 * MMAX + q2] = s2 * t_pp + c2 * t_pq;
+          if (q1 >= 0) A[q1 * MMAX + p2] = c2 * t_qp - s2 * t_qq;
+          if (q1 >= 0 && q2 >= 0) A[q1 * MMAX + q2] = s2 * t_qp + c2 * t_qq;
+        } else {                            // V row j, pair b
+          const int f = e - nA;
+          const int j = f / npair, ib = f - j * npair;
+          const int p = rp[ib], q = rq[ib];
+          if (q < 0 || rs[ib] == 0.0) continue;
+          const double c = rc[ib], sn = rs[ib];
+          const double vp = V[j * MMAX + p], vq = V[j * MMAX + q];
+          V[j * MMAX + p] = c * vp - sn * vq;
+          V[j * MMAX + q] = sn * vp + c * vq;
+        }
       }
       __syncthreads();
     }
+    const int rot = s_rot;
+    __syncthreads();                        // every thread has read the flag before its reset
+    if (!rot) break;
   }
-  __syncthreads();
 }
 
 // Generalised Rayleigh-Ritz: GA c = lam GB c, k smallest lam.  The basis is
@@ -191,9 +256,10 @@ __device__ void jacobi_eig(double* A, double* V, int m) {
 //   Q = V_keep diag(s^-1/2), H = Q^T (D GB D) Q = Y diag(mu) Y^T,
 //   lam = 1/mu for the k largest mu, c = D Q y / sqrt(mu)  (M-normalised).
 // Out (global): C (m x k, row-major), lam (k), status (0 ok, 1 failure).
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(RR_THREADS)
 k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__ C,
-                double* __restrict__ lam, double* __restrict__ status, double drop_tol) {
+                double* __restrict__ lam, double* __restrict__ status, double drop_tol,
+                double jeps) {
   __shared__ double A[MMAX * MMAX], B[MMAX * MMAX], V[MMAX * MMAX], T1[MMAX * MMAX];
   __shared__ double dsc[MMAX], sig[MMAX], mu[MMAX];
   __shared__ int keep[MMAX], perm[MMAX];
@@ -213,7 +279,7 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
     B[i * MMAX + j] = 0.5 * (GB[i * m + j] + GB[j * m + i]) * sc;
   }
   __syncthreads();
-  jacobi_eig(A, V, m);
+  jacobi_eig(A, V, m, jeps);
   if (tid == 0) {
     double smax = 0.0;
     for (int i = 0; i < m; ++i) { sig[i] = A[i * MMAX + i]; smax = fmax(smax, sig[i]); }
@@ -248,7 +314,7 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
     B[l * MMAX + q] = 0.5 * (A[l * MMAX + q] + A[q * MMAX + l]);
   }
   __syncthreads();
-  jacobi_eig(B, V, r);                                      // V: eigenvectors Y (r x r)
+  jacobi_eig(B, V, r, jeps);                                // V: eigenvectors Y (r x r)
   if (tid == 0) {
     for (int i = 0; i < r; ++i) { mu[i] = B[i * MMAX + i]; perm[i] = i; }
     for (int i = 0; i < r; ++i) {                           // selection sort, descending mu
@@ -390,18 +456,21 @@ static unsigned grid_blocks(int64_t n) {
 }
 
 static int gram_and_rr(LobpcgWork& w, int m, int k, hipStream_t st, double drop_tol) {
-  const int nt = (m + GT - 1) / GT;
-  hipLaunchKernelGGL(k_gram, dim3(w.gram_chunks, 2 * nt * nt), dim3(256), 0, st, w.n, m, w.S,
+  hipLaunchKernelGGL(k_gram, dim3(w.gram_chunks), dim3(256), gram_lds_bytes(m), st, w.n, m, w.S,
                      w.AS, w.BS, w.gram_partials);
   VOX8_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_gram_finish, dim3(8), dim3(256), 0, st, w.gram_chunks, m,
+  hipLaunchKernelGGL(k_gram_finish, dim3(2 * m * m), dim3(64), 0, st, w.gram_chunks, m,
                      (const double*)w.gram_partials, w.G);
   VOX8_HIP(hipGetLastError());
   double* C = w.G + 2 * MMAX * MMAX;
   double* lam = C + MMAX * 12;
   double* status = lam + 12;
-  hipLaunchKernelGGL(k_rayleigh_ritz, dim3(1), dim3(64), 0, st, m, k, (const double*)w.G, C, lam,
-                     status, drop_tol);
+  static const double jeps = [] {
+    const char* e = getenv("VOX8_JACOBI_EPS");
+    return e ? atof(e) : 1e-18;
+  }();
+  hipLaunchKernelGGL(k_rayleigh_ritz, dim3(1), dim3(RR_THREADS), 0, st, m, k, (const double*)w.G, C, lam,
+                     status, drop_tol, jeps);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

@@ -212,24 +212,30 @@ __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __res
     b[i] = (float)a[i];
 }
 
-// x = Ainv b per column; Ainv symmetric, read column-wise (coalesced over rows);
-// 64 rows x 4 partial sums per block, combined in LDS.
+// x = Ainv b per column (Ainv symmetric).  Each wavefront owns 4 rows; its
+// lanes stride over the n columns (coalesced row reads), then reduce across
+// the wavefront.  16 rows per block, fixed summation order.
 __global__ void __launch_bounds__(256)
 k_mg_coarse_apply(int n, const float* __restrict__ Ainv, const float* __restrict__ b,
                   float* __restrict__ x) {
-  __shared__ float part[4][64];
-  const int row = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int grp = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int row0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
   const int col = blockIdx.y;
   const float* bc = b + (int64_t)col * n;
-  float s = 0.0f;
-  if (row < n)
-    for (int j = grp; j < n; j += 4) s += Ainv[(int64_t)j * n + row] * bc[j];
-  part[grp][threadIdx.x & 63] = s;
-  __syncthreads();
-  if (grp == 0 && row < n)
-    x[(int64_t)col * n + row] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                               part[3][threadIdx.x];
+  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k = lane; k < n; k += 64) {
+    const float bk = bc[k];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (row0 + i < n) s[i] += Ainv[(int64_t)(row0 + i) * n + k] * bk;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float v = s[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0 && row0 + i < n) x[(int64_t)col * n + row0 + i] = v;
+  }
 }
 
 __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -531,7 +537,7 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
   }
   if (mg.n_coarse) {
     MGLevel& C = mg.lv.back();
-    hipLaunchKernelGGL(k_mg_coarse_apply, dim3((mg.n_coarse + 63) / 64, ncols), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_mg_coarse_apply, dim3((mg.n_coarse + 15) / 16, ncols), dim3(256), 0, st,
                        mg.n_coarse, (const float*)mg.Ainv, (const float*)C.b, C.x);
     VOX8_HIP(hipGetLastError());
   }

@@ -15,19 +15,28 @@
 // verifies the structure; otherwise the general dense kernel is used.
 //
 // Data movement: one workgroup owns a (TX-1) x (TY-1) node tile of an x-y
-// plane and marches over a z range.  Per element layer it stages one new node
-// plane (with a one-node halo) in LDS, computes the TX x TY elements touching
-// the tile (one per thread), writes their 24 nodal forces to LDS
-// (component-major, conflict-free), and every owned node gathers its four
-// bottom-face and four top-face contributions.  Each input value is read from
-// HBM once per tile (plus the halo), each output written once, in x-contiguous
-// rows.  Summation order is fixed, so results are bitwise reproducible.
+// plane and marches over a z range; thread (tx, ty) computes element
+// (tx, ty) of the tile in every element layer.  A wavefront is one element row
+// (TX = 64), so the x-neighbour exchange of nodal forces is a lane shuffle; the
+// y-neighbour exchange goes through LDS (6 values per element and operator);
+// the z-neighbour exchange stays in registers (the top face of layer L is
+// carried to the bottom face of layer L+1).  Node planes (with a one-node halo)
+// are staged in LDS, the next one prefetched into registers while the current
+// layer computes.  Inputs are read once per tile plus the halo, outputs written
+// once; the summation order is fixed, so results are bitwise reproducible.
+// Tiles are numbered so that each XCD (workgroups are dealt round-robin to the
+// 8 XCDs) receives a contiguous range, keeping halo re-reads in its L2.
+#include <cstdlib>
+
 #include "vox8_fast.h"
 
 namespace vox8 {
 
-constexpr int TX = 32, TY = 8, NT = TX * TY;            // element tile = threads
+constexpr int TX = 64, TY = 8, NT = TX * TY;            // element tile = threads
+constexpr int OX = TX - 1, OY = TY - 1;                 // owned nodes per tile
 constexpr int PX = TX + 1, PY = TY + 1, PN = PX * PY;   // node plane incl. halo
+constexpr int NLD = (PN + NT - 1) / NT;                 // plane nodes loaded per thread
+constexpr int NXCD = 8;
 
 template <typename T, int PAT>
 __device__ __forceinline__ void block_products(const FastCoef<T>& C, const T al[9], const T ab[9],
@@ -100,19 +109,119 @@ __device__ __forceinline__ void inv8(const T m[8], T v[8]) {
   v[4] = s2 - d2; v[5] = s2 + d2; v[6] = s3 - d3; v[7] = s3 + d3;
 }
 
+// corner k = ox + 2 oy + 4 oz of the element.  After the x exchange a thread
+// holds, per face oz and row oy, the sum of its element's ox = 1 corner and the
+// right neighbour's ox = 0 corner: s[oy][oz] belongs to node (ex + 1, ey + oy).
+template <typename T>
+__device__ __forceinline__ void x_exchange(const T f[3][8], T s[2][2][3]) {
+#pragma unroll
+  for (int oz = 0; oz < 2; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 2; ++oy)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int k0 = 2 * oy + 4 * oz;
+        s[oy][oz][c] = f[c][k0 + 1] + __shfl_down(f[c][k0], 1, 64);
+      }
+}
+
+// Element forces of one operator for the thread's element (monomials in, corner
+// forces out).
+template <typename T, int PAT>
+__device__ __forceinline__ void element_K(const FastCoef<T>& C, T fk, const T mono[3][8], T f[3][8]) {
+  if (PAT == PAT_CUBE) {
+    // class values scaled by the element factor; al/ab index 3 * monomial + c
+    const T A = fk * C.lin[0][0], B = fk * C.lin[0][4], Cs = fk * C.lin[1][1];
+    const T D = fk * C.bil[0][0], E = fk * C.bil[0][5], F = fk * C.bil[2][2], G = fk * C.bil[2][3];
+    const T Hs = fk * C.tri[0][0];
+    // lin block: normal terms (xi_x, eta_y, zeta_z), shear pairs
+    const T a0 = mono[0][1], a4 = mono[1][2], a8 = mono[2][3];
+    const T S = a0 + a4 + a8, AB = A - B, BS = B * S;
+    const T l0 = AB * a0 + BS, l4 = AB * a4 + BS, l8 = AB * a8 + BS;
+    const T l13 = Cs * (mono[1][1] + mono[0][2]);   // (eta_x... ) pair: gl[1] = gl[3]
+    const T l26 = Cs * (mono[2][1] + mono[0][3]);   // gl[2] = gl[6]
+    const T l57 = Cs * (mono[2][2] + mono[1][3]);   // gl[5] = gl[7]
+    // bil block (ab[3 m + c] = mono[c][4 + m])
+    const T b0 = mono[0][4], b1 = mono[1][4], b2 = mono[2][4];
+    const T b3 = mono[0][5], b4 = mono[1][5], b5 = mono[2][5];
+    const T b6 = mono[0][6], b7 = mono[1][6], b8 = mono[2][6];
+    const T g0 = D * b0 + E * b5, g5 = E * b0 + D * b5;
+    const T g1 = D * b1 + E * b8, g8 = E * b1 + D * b8;
+    const T g4 = D * b4 + E * b6, g6 = E * b4 + D * b6;
+    const T FG = F - G, GS = G * (b2 + b3 + b7);
+    const T g2 = FG * b2 + GS, g3 = FG * b3 + GS, g7 = FG * b7 + GS;
+    T m[8];
+    // component x: lin rows 0, 3, 6; bil rows 0, 3, 6; tri row 0
+    m[0] = T(0); m[1] = l0; m[2] = l13; m[3] = l26; m[4] = g0; m[5] = g3; m[6] = g6;
+    m[7] = Hs * mono[0][7];
+    inv8(m, f[0]);
+    // component y: lin rows 1, 4, 7; bil rows 1, 4, 7; tri row 1
+    m[0] = T(0); m[1] = l13; m[2] = l4; m[3] = l57; m[4] = g1; m[5] = g4; m[6] = g7;
+    m[7] = Hs * mono[1][7];
+    inv8(m, f[1]);
+    // component z: lin rows 2, 5, 8; bil rows 2, 5, 8; tri row 2
+    m[0] = T(0); m[1] = l26; m[2] = l57; m[3] = l8; m[4] = g2; m[5] = g5; m[6] = g8;
+    m[7] = Hs * mono[2][7];
+    inv8(m, f[2]);
+    return;
+  }
+  T al[9], ab[9], at[3], gl[9], gb[9], gt[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    al[c] = mono[c][1]; al[3 + c] = mono[c][2]; al[6 + c] = mono[c][3];
+    ab[c] = mono[c][4]; ab[3 + c] = mono[c][5]; ab[6 + c] = mono[c][6];
+    at[c] = mono[c][7];
+  }
+  block_products<T, PAT>(C, al, ab, at, gl, gb, gt);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    T m[8];
+    m[0] = T(0);
+    m[1] = fk * gl[c]; m[2] = fk * gl[3 + c]; m[3] = fk * gl[6 + c];
+    m[4] = fk * gb[c]; m[5] = fk * gb[3 + c]; m[6] = fk * gb[6 + c];
+    m[7] = fk * gt[c];
+    inv8(m, f[c]);
+  }
+}
+
+template <typename T, int PAT>
+__device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mono[3][8], T f[3][8]) {
+  if (PAT == PAT_CUBE) {
+    const T w0 = fm * C.md[0], w1 = fm * C.md[1], w2 = fm * C.md[4], w3 = fm * C.md[7];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T m[8];
+      m[0] = w0 * mono[c][0];
+      m[1] = w1 * mono[c][1]; m[2] = w1 * mono[c][2]; m[3] = w1 * mono[c][3];
+      m[4] = w2 * mono[c][4]; m[5] = w2 * mono[c][5]; m[6] = w2 * mono[c][6];
+      m[7] = w3 * mono[c][7];
+      inv8(m, f[c]);
+    }
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    T m[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m[k] = fm * C.md[k] * mono[c][k];
+    inv8(m, f[c]);
+  }
+}
+
 // MODE (K only): EPI_PLAIN y = A x; EPI_RESID r = b - A x, d = c2 D^-1 r;
 // EPI_CHEB one Chebyshev smoothing step with x holding the direction d:
 //   xs += d, r -= A d, d' = c1 d + c2 D^-1 r   (all node-local once A d is known)
 template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE>
 __global__ void __launch_bounds__(NT)
-k_apply_fast(Grid g, int zchunk, FastCoef<T> C, const T* __restrict__ fK,
+k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const T* __restrict__ fK,
              const T* __restrict__ fM, const uint8_t* __restrict__ fixed,
              const T* __restrict__ x, int64_t ldx, T* __restrict__ yK, T* __restrict__ yM,
              int64_t ldy, DotFuse fuse, Epilogue<T> ep) {
-  __shared__ T pl[2][3][PN];        // two node planes (component-major)
-  __shared__ T buf[24][NT];          // nodal forces of the layer's elements
-  __shared__ double sh[8];
-  const int col = blockIdx.z;
+  constexpr int NOP = (DO_K ? 1 : 0) + (DO_M ? 1 : 0);
+  __shared__ T pl[2][3][PN];          // two node planes, component-major
+  __shared__ T xch[NOP][2][3][NT];     // y exchange: s[0][oz][c] per element
+  __shared__ double sh[NT / 64];
+  const int col = blockIdx.y;
   if (FUSE && fuse.scal[col].done) return;
   x += col * ldx;
   if (DO_K && MODE == EPI_PLAIN) yK += col * ldy;
@@ -123,49 +232,100 @@ k_apply_fast(Grid g, int zchunk, FastCoef<T> C, const T* __restrict__ fK,
     if (MODE == EPI_RESID) ep.b += col * ep.ld;
     if (MODE == EPI_CHEB) ep.xs += col * ep.ld;
   }
+  // XCD-aware tile number: XCD j = bid % 8 gets the contiguous range
+  // [j q + min(j, r), ...) of the nb = 8 q + r tiles
+  const int nb = gridDim.x, bid = blockIdx.x;
+  const int xq = nb / NXCD, xr = nb % NXCD, xj = bid % NXCD;
+  const int tile = xj * xq + min(xj, xr) + bid / NXCD;
+  const int tx_t = tile % tiles_x, rest = tile / tiles_x;
+  const int ty_t = rest % tiles_y, zc_t = rest / tiles_y;
   const int t = threadIdx.x;
   const int tx = t % TX, ty = t / TX;
-  // blockIdx.x: tile column; blockIdx.y: (tile row, z chunk); blockIdx.z: vector
-  const int ty_tiles = (g.NY + TY - 2) / (TY - 1);
-  const int nx0 = blockIdx.x * (TX - 1);                      // owned node origin
-  const int oy0 = (blockIdx.y % ty_tiles) * (TY - 1);
-  const int zs = (blockIdx.y / ty_tiles) * zchunk;
+  const int nx0 = tx_t * OX, oy0 = ty_t * OY;                  // owned node origin
+  const int zs = zc_t * zchunk;
   const int ze = min(zs + zchunk, g.NZ);                       // owned node planes [zs, ze)
   const int ex0 = nx0 - 1, ey0 = oy0 - 1;                      // element tile origin
-  // owned node of this thread (gather role)
-  const bool owner = tx < TX - 1 && ty < TY - 1;
-  const int gx = nx0 + tx, gy = oy0 + ty;
+  const bool owner = tx < OX && ty < OY;
+  const int gx = nx0 + tx, gy = oy0 + ty;                      // owned node
   const bool own_in = owner && gx < g.NX && gy < g.NY;
+  const int ex = ex0 + tx, ey = ey0 + ty;                      // element
+  const bool exy_in = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
 
-  auto load_plane = [&](int slot, int iz) {
-    for (int q = t; q < 3 * PN; q += NT) {
-      const int row = q / (3 * PX), rem = q % (3 * PX);
-      const int px = rem / 3, c = rem % 3;
-      const int nxg = ex0 + px, nyg = ey0 + row;
-      T v = T(0);
-      if (iz >= 0 && iz < g.NZ && nxg >= 0 && nxg < g.NX && nyg >= 0 && nyg < g.NY) {
-        const int64_t n = nxg + (int64_t)g.NX * (nyg + (int64_t)g.NY * iz);
-        if (!((fixed[n] >> c) & 1)) v = x[3 * n + c];
+  // plane staging: thread loads nodes q = t + i NT of the (PX x PY) plane.
+  // Offsets are 32-bit (the launcher checks 3 n_nodes < 2^31); the in-plane
+  // part is fixed per thread, the plane part advances by NX NY per layer.
+  const uint32_t plane = (uint32_t)g.NX * (uint32_t)g.NY;
+  uint32_t qoff[NLD];
+  bool qin[NLD];
+#pragma unroll
+  for (int i = 0; i < NLD; ++i) {
+    const int q = t + i * NT;
+    const int py = q / PX, px = q - py * PX;
+    const int nxg = ex0 + px, nyg = ey0 + py;
+    qin[i] = q < PN && nxg >= 0 && nxg < g.NX && nyg >= 0 && nyg < g.NY;
+    qoff[i] = qin[i] ? (uint32_t)nxg + (uint32_t)g.NX * (uint32_t)nyg : 0u;
+  }
+  T pre[NLD][3];
+  auto fetch = [&](int iz) {
+    const bool zin = iz >= 0 && iz < g.NZ;
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      T v0 = T(0), v1 = T(0), v2 = T(0);
+      if (zin && qin[i]) {
+        const uint32_t n = qoff[i] + plane * (uint32_t)iz;
+        const uint32_t f = fixed[n];
+        v0 = x[3u * n];
+        v1 = x[3u * n + 1u];
+        v2 = x[3u * n + 2u];
+        v0 = (f & 1u) ? T(0) : v0;
+        v1 = (f & 2u) ? T(0) : v1;
+        v2 = (f & 4u) ? T(0) : v2;
       }
-      pl[slot][c][row * PX + px] = v;
+      pre[i][0] = v0; pre[i][1] = v1; pre[i][2] = v2;
     }
   };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      const int q = t + i * NT;
+      if (q < PN)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pl[slot][c][q] = pre[i][c];
+    }
+  };
+  const uint32_t exy = exy_in ? (uint32_t)ex + (uint32_t)g.nx * (uint32_t)ey : 0u;
+  const uint32_t eplane = (uint32_t)g.nx * (uint32_t)g.ny;
+  const uint32_t nxy = own_in ? (uint32_t)gx + (uint32_t)g.NX * (uint32_t)gy : 0u;
 
-  T partK[3] = {0, 0, 0}, partM[3] = {0, 0, 0};
+  T carK[3] = {0, 0, 0}, carM[3] = {0, 0, 0};
   double dot = 0.0;
-  int cur = 0;
-  load_plane(cur, zs - 1);
+  fetch(zs - 1);
+  store(0);
+  fetch(zs);
+  store(1);
+  __syncthreads();
   for (int L = zs - 1; L < ze; ++L) {
-    load_plane(cur ^ 1, L + 1);
-    __syncthreads();
-    // ---- element of this thread in layer L ----
-    const int ex = ex0 + tx, ey = ey0 + ty;
-    const bool elem_in = L >= 0 && L < g.nz && ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
+    const int s0 = (L - zs + 1) & 1;                            // slot of plane L
+    if (L + 2 <= ze) fetch(L + 2);
     T fk = T(0), fm = T(0);
-    if (elem_in) {
-      const int64_t e = ex + (int64_t)g.nx * (ey + (int64_t)g.ny * L);
+    if (exy_in && L >= 0 && L < g.nz) {
+      const uint32_t e = exy + eplane * (uint32_t)L;
       if (DO_K) fk = fK[e];
       if (DO_M) fm = fM[e];
+    }
+    // owned-node operands of the epilogue (plane L), loaded early
+    const bool out_ok = own_in && L >= zs;
+    const uint32_t nn = nxy + plane * (uint32_t)(L < 0 ? 0 : L);
+    T eb[3] = {0, 0, 0}, er[3] = {0, 0, 0}, ed[3] = {0, 0, 0};
+    uint32_t fnode = 0;
+    if (out_ok) {
+      fnode = fixed[nn];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (MODE == EPI_RESID) eb[c] = ep.b[3u * nn + c];
+        if (MODE == EPI_CHEB) er[c] = ep.r[3u * nn + c];
+        if (MODE != EPI_PLAIN) ed[c] = ep.dinv[3u * nn + c];
+      }
     }
     T mono[3][8];
 #pragma unroll
@@ -174,116 +334,77 @@ k_apply_fast(Grid g, int zchunk, FastCoef<T> C, const T* __restrict__ fK,
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-        v[k] = pl[oz ? (cur ^ 1) : cur][c][(ty + oy) * PX + tx + ox];
+        v[k] = pl[s0 ^ oz][c][(ty + oy) * PX + tx + ox];
       }
       fwd8(v, mono[c]);
     }
-    // ---- K part ----
+    T xin[3];                                                   // owned node's input value
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xin[c] = pl[s0][c][(ty + 1) * PX + tx + 1];
+    T sK[2][2][3], sM[2][2][3];
     if (DO_K) {
-      T al[9], ab[9], at[3], gl[9], gb[9], gt[3];
+      T f[3][8];
+      element_K<T, PAT>(C, fk, mono, f);
+      x_exchange(f, sK);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        al[c] = mono[c][1]; al[3 + c] = mono[c][2]; al[6 + c] = mono[c][3];
-        ab[c] = mono[c][4]; ab[3 + c] = mono[c][5]; ab[6 + c] = mono[c][6];
-        at[c] = mono[c][7];
-      }
-      block_products<T, PAT>(C, al, ab, at, gl, gb, gt);
+      for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T m[8], v[8];
-        m[0] = T(0);
-        m[1] = fk * gl[c]; m[2] = fk * gl[3 + c]; m[3] = fk * gl[6 + c];
-        m[4] = fk * gb[c]; m[5] = fk * gb[3 + c]; m[6] = fk * gb[6 + c];
-        m[7] = fk * gt[c];
-        inv8(m, v);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) buf[3 * k + c][t] = v[k];
-      }
-      __syncthreads();
-      if (owner) {
-        T bot[3] = {partK[0], partK[1], partK[2]}, top[3] = {0, 0, 0};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int sx = s & 1, sy = s >> 1;
-          const int el = (ty + sy) * TX + tx + sx;
-          const int cb = (1 - sx) + 2 * (1 - sy);      // corner of the node in that element
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            bot[c] += buf[3 * cb + c][el];
-            top[c] += buf[3 * (cb + 4) + c][el];
-          }
-        }
-        if (L >= zs && own_in) {
-          const int64_t n = gx + (int64_t)g.NX * (gy + (int64_t)g.NY * L);
-          const uint8_t f = fixed[n];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const bool fx = (f >> c) & 1;
-            const T val = fx ? T(0) : bot[c];
-            const T xin = pl[cur][c][(ty + 1) * PX + tx + 1];
-            if (MODE == EPI_PLAIN) {
-              yK[3 * n + c] = val;
-              if (FUSE) dot += (double)val * (double)xin;
-            } else if (MODE == EPI_RESID) {
-              const T r = fx ? T(0) : ep.b[3 * n + c] - val;
-              ep.r[3 * n + c] = r;
-              ep.dnew[3 * n + c] = ep.c2 * ep.dinv[3 * n + c] * r;
-            } else {
-              ep.xs[3 * n + c] += xin;
-              const T r = ep.r[3 * n + c] - val;
-              ep.r[3 * n + c] = r;
-              ep.dnew[3 * n + c] = ep.c1 * xin + ep.c2 * ep.dinv[3 * n + c] * r;
-            }
-          }
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) partK[c] = top[c];
-      }
-      if (DO_M) __syncthreads();
+        for (int c = 0; c < 3; ++c) xch[0][oz][c][t] = sK[0][oz][c];
     }
-    // ---- M part (diagonal in the monomial basis) ----
     if (DO_M) {
+      T f[3][8];
+      element_M<T, PAT>(C, fm, mono, f);
+      x_exchange(f, sM);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        T m[8], v[8];
+      for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) m[k] = fm * C.md[k] * mono[c][k];
-        inv8(m, v);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) buf[3 * k + c][t] = v[k];
-      }
-      __syncthreads();
-      if (owner) {
-        T bot[3] = {partM[0], partM[1], partM[2]}, top[3] = {0, 0, 0};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int sx = s & 1, sy = s >> 1;
-          const int el = (ty + sy) * TX + tx + sx;
-          const int cb = (1 - sx) + 2 * (1 - sy);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            bot[c] += buf[3 * cb + c][el];
-            top[c] += buf[3 * (cb + 4) + c][el];
-          }
-        }
-        if (L >= zs && own_in) {
-          const int64_t n = gx + (int64_t)g.NX * (gy + (int64_t)g.NY * L);
-          const uint8_t f = fixed[n];
-#pragma unroll
-          for (int c = 0; c < 3; ++c) yM[3 * n + c] = ((f >> c) & 1) ? T(0) : bot[c];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) partM[c] = top[c];
-      }
+        for (int c = 0; c < 3; ++c) xch[NOP - 1][oz][c][t] = sM[0][oz][c];
     }
     __syncthreads();
-    cur ^= 1;
+    if (owner) {
+      const int tn = t + TX;                                    // element (tx, ty + 1)
+      T bK[3], bM[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (DO_K) {
+          bK[c] = carK[c] + sK[1][0][c] + xch[0][0][c][tn];
+          carK[c] = sK[1][1][c] + xch[0][1][c][tn];
+        }
+        if (DO_M) {
+          bM[c] = carM[c] + sM[1][0][c] + xch[NOP - 1][0][c][tn];
+          carM[c] = sM[1][1][c] + xch[NOP - 1][1][c][tn];
+        }
+      }
+      if (out_ok) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const bool fx = (fnode >> c) & 1;
+          if (DO_K) {
+            const T val = fx ? T(0) : bK[c];
+            if (MODE == EPI_PLAIN) {
+              yK[3u * nn + c] = val;
+              if (FUSE) dot += (double)val * (double)xin[c];
+            } else if (MODE == EPI_RESID) {
+              const T r = fx ? T(0) : eb[c] - val;
+              ep.r[3u * nn + c] = r;
+              ep.dnew[3u * nn + c] = ep.c2 * ed[c] * r;
+            } else {
+              ep.xs[3u * nn + c] += xin[c];
+              const T r = er[c] - val;
+              ep.r[3u * nn + c] = r;
+              ep.dnew[3u * nn + c] = ep.c1 * xin[c] + ep.c2 * ed[c] * r;
+            }
+          }
+          if (DO_M) yM[3u * nn + c] = fx ? T(0) : bM[c];
+        }
+      }
+    }
+    if (L + 2 <= ze) store(s0);                                 // plane L + 2 replaces plane L
+    __syncthreads();
   }
   if (FUSE) {
     const double v = block_sum(dot, sh);
     double tot;
-    const int nb = gridDim.x * gridDim.y;
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     if (grid_reduce_last<1>(&v, fuse.partials + (int64_t)col * nb, fuse.counters + col, nb, bid,
                             &tot, sh)) {
       if (threadIdx.x == 0) fuse.scal[col].pq = tot;
@@ -291,35 +412,51 @@ k_apply_fast(Grid g, int zchunk, FastCoef<T> C, const T* __restrict__ fK,
   }
 }
 
+static int zchunk_override() {
+  static const int v = [] {
+    const char* e = getenv("VOX8_ZCHUNK");
+    return e ? atoi(e) : 0;
+  }();
+  return v;
+}
+
+static int tiles_x(const Grid& g) { return (g.NX + OX - 1) / OX; }
+static int tiles_y(const Grid& g) { return (g.NY + OY - 1) / OY; }
+
 int fast_zchunk(const Grid& g) {
-  // enough blocks to fill 256 CUs several times over; at least 4 planes per chunk
-  const int64_t tiles = (int64_t)((g.NX + TX - 2) / (TX - 1)) * ((g.NY + TY - 2) / (TY - 1));
+  // z planes per workgroup: halve until the launch has enough workgroups to fill
+  // 256 CUs (at least 4 planes, so the z halo stays a modest overhead)
+  const int ov = zchunk_override();
+  if (ov > 0) return ov < g.NZ ? ov : g.NZ;
+  const int64_t tiles = (int64_t)tiles_x(g) * tiles_y(g);
   int z = g.NZ;
-  while (z > 4 && tiles * ((g.NZ + z - 1) / z) < 2048) z = (z + 1) / 2;
+  while (z > 4 && tiles * ((g.NZ + z - 1) / z) < 384) z = (z + 1) / 2;
   return z;
 }
 
 int fast_blocks(const Grid& g) {
   const int zc = fast_zchunk(g);
-  return ((g.NX + TX - 2) / (TX - 1)) * ((g.NY + TY - 2) / (TY - 1)) * ((g.NZ + zc - 1) / zc);
+  return tiles_x(g) * tiles_y(g) * ((g.NZ + zc - 1) / zc);
 }
 
 template <typename T>
 int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,
                       const T* fM, const uint8_t* fixed, const T* x, int64_t ldx, T* yK, T* yM,
                       int64_t ldy, int nrhs, hipStream_t st, DotFuse fuse, Epilogue<T> ep) {
+  if (3 * g.n_nodes >= ((int64_t)1 << 31)) {
+    set_error("structured stencil: grid too large for 32-bit offsets (use the dense kernel)");
+    return ERR_ARG;
+  }
   const int zc = fast_zchunk(g);
-  const int gx = (g.NX + TX - 2) / (TX - 1);
-  const int gy = ((g.NY + TY - 2) / (TY - 1)) * ((g.NZ + zc - 1) / zc);
-  const dim3 grid(gx, gy, nrhs), blk(NT);
+  const dim3 grid(fast_blocks(g), nrhs), blk(NT);
   const bool fz = fuse.scal != nullptr;
   if (ep.mode != EPI_PLAIN && (op != OP_K || fz)) {
     set_error("stencil epilogue modes apply to K alone");
     return ERR_ARG;
   }
 #define VOX8_LAUNCH(PAT, K, M, F, MD)                                                            \
-  hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD>), grid, blk, 0, st, g, zc, C, fK, fM, fixed, \
-                     x, ldx, yK, yM, ldy, fuse, ep)
+  hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD>), grid, blk, 0, st, g, zc, tiles_x(g),   \
+                     tiles_y(g), C, fK, fM, fixed, x, ldx, yK, yM, ldy, fuse, ep)
 #define VOX8_PAT(PAT)                                                                   \
   if (op == OP_K) {                                                                     \
     if (ep.mode == EPI_RESID) VOX8_LAUNCH(PAT, true, false, false, EPI_RESID);          \
@@ -328,7 +465,9 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     else VOX8_LAUNCH(PAT, true, false, false, EPI_PLAIN);                               \
   } else if (op == OP_M) VOX8_LAUNCH(PAT, false, true, false, EPI_PLAIN);                \
   else VOX8_LAUNCH(PAT, true, true, false, EPI_PLAIN);
-  if (pattern == PAT_ORTHO) { VOX8_PAT(PAT_ORTHO) } else { VOX8_PAT(PAT_BLOCK) }
+  if (pattern == PAT_CUBE) { VOX8_PAT(PAT_CUBE) }
+  else if (pattern == PAT_ORTHO) { VOX8_PAT(PAT_ORTHO) }
+  else { VOX8_PAT(PAT_BLOCK) }
 #undef VOX8_PAT
 #undef VOX8_LAUNCH
   VOX8_HIP(hipGetLastError());
@@ -417,6 +556,21 @@ int fast_coefficients(const double* Ke, const double* Me, FastCoef<double>* c64,
   for (int r = 0; r < 3; ++r)
     for (int s = 0; s < 3; ++s)
       if (r != s && fabs(C.tri[r][s]) > tk) pat = PAT_BLOCK;
+  if (pat == PAT_ORTHO) {
+    // cubic isotropy: every entry equals its class representative and the
+    // mass diagonal depends on the monomial degree only
+    double vals[45];
+    for (int i = 0; i < 21; ++i) vals[i] = C.lin[ORTHO_LIN[i][0]][ORTHO_LIN[i][1]];
+    for (int i = 0; i < 21; ++i) vals[21 + i] = C.bil[ORTHO_BIL[i][0]][ORTHO_BIL[i][1]];
+    for (int i = 0; i < 3; ++i) vals[42 + i] = C.tri[i][i];
+    const double rep[8] = {C.lin[0][0], C.lin[0][4], C.lin[1][1], C.bil[0][0],
+                           C.bil[0][5], C.bil[2][2], C.bil[2][3], C.tri[0][0]};
+    bool cube = true;
+    for (int i = 0; i < 45; ++i) cube = cube && fabs(vals[i] - rep[CUBE_CLASS[i]]) <= tk;
+    for (int k = 1; k < 3; ++k) cube = cube && fabs(C.md[1 + k] - C.md[1]) <= tm &&
+                                       fabs(C.md[4 + k] - C.md[4]) <= tm;
+    if (cube) pat = PAT_CUBE;
+  }
   *c64 = C;
   for (int r = 0; r < 9; ++r)
     for (int s = 0; s < 9; ++s) {
