@@ -1,17 +1,26 @@
 """Benchmark: SIMP iterations/s of the voxel hex8 FEM hot path on one MI355X.
 
-One step = one full SIMP iteration of the workload (default c2: 220x32x24
-voxels at 1 mm, 546,975 DOF): density filter, Jacobi-PCG static solve (rtol
-1e-8, warm-started), compliance adjoint, 6-mode LOBPCG (tol 1e-6,
-warm-started), mode-1 eigenvalue adjoint (the frequency constraint is set
-above f1 so it is active every iteration), sensitivity filters, OC update.
-Inputs are resident in HBM before the timed region.
+One step = one full SIMP iteration of the workload (default c5t: 440x64x48
+voxels at 0.5 mm, 4,213,755 DOF, 5 load cases, 6 modes): density filter, the
+5-load-case static solve as one block PCG (rtol 1e-10, warm-started),
+compliance adjoint, 6-mode LOBPCG (tol 1e-6, warm-started), mode-1
+eigenvalue adjoint (the frequency target is above f1, so the constraint is
+active every iteration), sensitivity filters, OC update.  Inputs are resident
+in HBM before the timed region.
 
-Prints one JSON line (rank 0).  Extra objects: "roofline" for the stencil
-K.u (HIP-event timing of the dominant kernel, algorithmic bytes
-6*8*N_nodes + 8*N_elem per FP64 launch), "plain_fp64" (the same loop on the
-plain FP64 path) and "cpu_baseline" (the SciPy oracle on this host, bounded
-sample, extrapolated per SIMP iteration with the GPU run's iteration counts).
+Prints one JSON line.  Extra objects:
+  roofline      the dominant kernel (FP32 Chebyshev smoothing step of the
+                multigrid preconditioner on the fine grid, 6 columns), timed
+                with HIP events on the solver stream; algorithmic bytes per
+                launch in "alg_bytes_per_launch"; "traffic" from the committed
+                rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
+  kernels       the same measurement for the other hot kernels
+  plain_fp64    one SIMP iteration of the same loop on the plain FP64 path
+                (the reference's Jacobi preconditioner in PCG and LOBPCG)
+  secondary     the other workloads given with --also (default c2)
+  cpu_baseline  the SciPy oracle on this host (bounded z-slab sample of the
+                same grid, scaled by DOF count and by the plain path's
+                iteration counts, i.e. the reference's algorithm)
 """
 from __future__ import annotations
 
@@ -28,10 +37,10 @@ for _p in (ROOT, ROOT / "to-f450-arm_amd"):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0
+DOMINANT = "cheb_fp32"
 
 
 def make_optimizer(wl_name, precision, modal_maxiter=2000):
-    import numpy as np
     from vox8.fem import FEM3D, Material
     from vox8.topopt import SIMPOptimizer
     from vox8.workloads import E_PA, NU, RHO_MAT, WORKLOADS, load_cases, seeded_density
@@ -46,7 +55,7 @@ def make_optimizer(wl_name, precision, modal_maxiter=2000):
                         r_min_mm=wl.r_min_vox * geom.dx * 1e3, max_iter=10 ** 6,
                         min_freq_Hz=1e4, tip_mass_kg=0.0, n_modes_track=wl.n_modes,
                         F_list=list(F), solve_rtol=wl.pcg_rtol, modal_tol=wl.modal_tol,
-                        modal_maxiter=modal_maxiter, solve_maxiter=20000)
+                        modal_maxiter=modal_maxiter, solve_maxiter=40000)
     opt.start(rho_init=x0)
     return wl, geom, fem, opt, F, x0
 
@@ -63,58 +72,97 @@ def time_steps(opt, warmup, steps):
     return time.perf_counter() - t0
 
 
-def stencil_roofline(fem, geom, reps=50):
-    """HIP-event timing of the FP64 single-RHS K.u launch on the solver stream."""
+def _event_time(fem, fn, reps):
+    """Average time of fn() on the solver's stream (HIP events on that stream)."""
     import torch
-    from vox8 import _lib
-    n = geom.n_dofs
-    x = torch.randn(n, dtype=torch.float64, device=fem.device)
-    y = torch.empty_like(x)
-    rho = torch.full((geom.n_elements,), 0.3, dtype=torch.float64, device=fem.device)
-    fem._set_density(rho)
-    args = (_lib.OP_K, _lib.PREC_FP64, _lib.ptr(x), _lib.ptr(y), None, 1, n)
-    for _ in range(5):
-        fem._call("vox8_apply", *args)
-    st = torch.cuda.current_stream()
+    for _ in range(3):
+        fn()
+    st = torch.cuda.current_stream()          # fem._call binds the library to this stream
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
     for _ in range(reps):
-        fem._call("vox8_apply", *args)
+        fn()
     e1.record(st)
     e1.synchronize()
-    t = e0.elapsed_time(e1) / 1e3 / reps
-    alg_bytes = 6 * 8 * geom.n_nodes + 8 * geom.n_elements
-    achieved = alg_bytes / t / 1e9
-    return {"kernel": "k_apply_dense<double,K> (K.u, FP64, 1 RHS)", "bound": "hbm",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(),
-            "avg_launch_us": round(t * 1e6, 2), "alg_bytes_per_launch": alg_bytes}
+    return e0.elapsed_time(e1) / 1e3 / reps
 
 
-def load_traffic():
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
-    p = ROOT / "profiles" / "stencil_Ku_fp64_c2_pmc.json"
-    if p.exists():
-        try:
-            return json.loads(p.read_text()).get("traffic_bytes")
-        except Exception:
-            return None
-    return None
+def kernel_rooflines(fem, geom, rho, nrhs=6, reps=30):
+    """HIP-event timings of the hot kernels at the workload's size and density."""
+    import torch
+    from vox8 import _lib
+    n, ne, nn = geom.n_dofs, geom.n_elements, geom.n_nodes
+    fem._set_density(rho)
+    out = {}
+    # FP32 Chebyshev smoothing step (multigrid fine level), nrhs columns
+    d = torch.randn(nrhs, n, dtype=torch.float32, device=fem.device)
+    r = torch.randn_like(d)
+    xs = torch.zeros_like(d)
+    dn = torch.empty_like(d)
+    t = _event_time(fem, lambda: fem._call("vox8_cheb_step", _lib.ptr(d), _lib.ptr(r), _lib.ptr(xs),
+                                           _lib.ptr(dn), 0.5, 1e-3, nrhs), reps)
+    alg = nrhs * 24 * n + 4 * n + 4 * ne + nn
+    out["cheb_fp32"] = dict(kernel=f"k_apply_fast<float,CUBE,K,EPI_CHEB> (FP32 Chebyshev step, {nrhs} columns)",
+                            t=t, alg=alg, per_unit="24 B per DOF and column (read d, r, xs; write r, xs, d'), "
+                                                   "+ 4 B/DOF D^-1, 4 B/element factor, 1 B/node mask")
+    # FP64 fused K and M products (LOBPCG), nrhs columns
+    x = torch.randn(nrhs, n, dtype=torch.float64, device=fem.device)
+    yk, ym = torch.empty_like(x), torch.empty_like(x)
+    t = _event_time(fem, lambda: fem._call("vox8_apply", _lib.OP_KM, _lib.PREC_FP64, _lib.ptr(x),
+                                           _lib.ptr(yk), _lib.ptr(ym), nrhs, n), reps)
+    alg = nrhs * 24 * n + 16 * ne + nn
+    out["km_fp64"] = dict(kernel=f"k_apply_fast<double,CUBE,K+M> (FP64 K and M products, {nrhs} columns)",
+                          t=t, alg=alg, per_unit="24 B per DOF and column (read x; write Kx, Mx), "
+                                                 "+ 16 B/element factors, 1 B/node mask")
+    # FP64 K product, one column (PCG matvec shape)
+    x1, y1 = x[0].contiguous(), yk[0].contiguous()
+    t = _event_time(fem, lambda: fem._call("vox8_apply", _lib.OP_K, _lib.PREC_FP64, _lib.ptr(x1),
+                                           _lib.ptr(y1), None, 1, n), reps)
+    alg = 16 * n + 8 * ne + nn
+    out["k_fp64"] = dict(kernel="k_apply_fast<double,CUBE,K> (FP64 K product, 1 column)", t=t, alg=alg,
+                         per_unit="16 B per DOF (read x; write Kx), + 8 B/element factor, 1 B/node mask")
+    res = {}
+    for k, v in out.items():
+        ach = v["alg"] / v["t"] / 1e9
+        res[k] = {"kernel": v["kernel"], "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic(k, geom),
+                  "avg_launch_us": round(v["t"] * 1e6, 2), "alg_bytes_per_launch": int(v["alg"]),
+                  "alg_bytes_per_unit": v["per_unit"]}
+    return res
 
 
-def cpu_baseline(wl, geom, F, x0, pcg_iters, lob_iters, budget_s=25.0):
-    """SciPy oracle on this host: per-phase timings on a bounded sample, combined
-    into seconds per SIMP iteration with the GPU run's mean iteration counts."""
+def load_traffic(name, geom):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
+    covers this kernel at this grid."""
+    p = ROOT / "profiles" / "pmc_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text()).get(f"{name}@{geom.nx}x{geom.ny}x{geom.nz}")
+        return None if d is None else d.get("traffic_bytes")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(wl, geom, pcg_iters, lob_iters, basis, slab_nz=6, budget_s=25.0):
+    """SciPy oracle on this host, timed on a z-slab of the workload's grid
+    (same voxel size, all nx x ny, slab_nz layers), scaled by the DOF ratio."""
+    import warnings
     import numpy as np
     import scipy.sparse.linalg as spla
     from oracle.fem import FEM3D as OFEM, Material as OMat
     from oracle.topopt import cone_kernel, filter_field, oc_update
-    from vox8.workloads import E_PA, NU, RHO_MAT
+    from vox8.workloads import E_PA, NU, RHO_MAT, Cantilever, load_cases, seeded_density
 
     cores = len(os.sched_getaffinity(0))
-    of = OFEM(geom, OMat(E=E_PA, nu=NU, rho=RHO_MAT))
-    g3 = lambda v: v.reshape(geom.nz, geom.ny, geom.nx).transpose(2, 1, 0)
-    mask = np.ones((geom.nx, geom.ny, geom.nz), dtype=bool)
+    nz = min(slab_nz, geom.nz)
+    sg = Cantilever(geom.nx, geom.ny, nz, geom.dx)
+    scale = geom.n_dofs / sg.n_dofs
+    of = OFEM(sg, OMat(E=E_PA, nu=NU, rho=RHO_MAT))
+    x0 = seeded_density(sg, wl.base, wl.amp, wl.seed)
+    F = load_cases(sg, wl.loads)
+    g3 = lambda v: v.reshape(sg.nz, sg.ny, sg.nx).transpose(2, 1, 0)
+    mask = np.ones((sg.nx, sg.ny, sg.nz), dtype=bool)
     ker = cone_kernel(wl.r_min_vox)
     t = time.perf_counter()
     phys = filter_field(g3(x0), mask, ker)
@@ -129,41 +177,54 @@ def cpu_baseline(wl, geom, F, x0, pcg_iters, lob_iters, budget_s=25.0):
     t_asm = time.perf_counter() - t
     d = Kff.diagonal()
     Minv = spla.LinearOperator(Kff.shape, matvec=lambda v: v / d, dtype=float)
-    n_cg = 0
+    n_cg = 60
     t = time.perf_counter()
-    spla.cg(Kff, F[0][free], M=Minv, rtol=1e-30, maxiter=40)
-    n_cg = 40
+    spla.cg(Kff, F[0][free], M=Minv, rtol=1e-30, maxiter=n_cg)
     t_cg = (time.perf_counter() - t) / n_cg
     X0 = np.random.default_rng(0).standard_normal((len(free), wl.n_modes))
     Tinv = spla.LinearOperator(Kff.shape, matvec=lambda v: v / d, matmat=lambda V: V / d[:, None],
                                dtype=float)
-    n_lob = 8
+    n_lob = 10
     t = time.perf_counter()
-    import warnings
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         spla.lobpcg(Kff, X0, B=Mff, M=Tinv, tol=1e-30, largest=False, maxiter=n_lob)
     t_lob = (time.perf_counter() - t) / n_lob
-    u = np.zeros(geom.n_dofs)
+    u = np.random.default_rng(1).standard_normal(sg.n_dofs)
     t = time.perf_counter()
     ce = of.elem_compliance(u)
     of.eigenvalue_sensitivity(rho, 1.0, u, tip_mass_kg=0.0)
     t_sens = time.perf_counter() - t
     t = time.perf_counter()
-    oc_update(x0, -np.abs(ce) - 1.0, geom.dx ** 3, 0.3 * x0.size * geom.dx ** 3, 0.2)
+    oc_update(x0, -np.abs(ce) - 1.0, sg.dx ** 3, wl.base * x0.size * sg.dx ** 3, 0.2)
     t_oc = time.perf_counter() - t
-    n_lc = F.shape[0]
-    per_iter = (2 * t_asm + n_lc * pcg_iters * t_cg + lob_iters * t_lob + t_sens
-                + 4 * t_filter + t_oc)
-    return {"value": round(1.0 / per_iter, 6), "unit": "SIMP iters/s", "cores": cores,
-            "kind": "port",
-            "sample": (f"SciPy oracle on {wl.name}: K/M COO->CSR assembly {t_asm:.2f}s, "
-                       f"{n_cg} CG iters ({t_cg * 1e3:.1f} ms/iter, CSR SpMV single-threaded), "
-                       f"{n_lob} LOBPCG iters ({t_lob * 1e3:.1f} ms/iter), einsum sensitivities "
-                       f"{t_sens:.2f}s, ndimage filter {t_filter:.2f}s x4, OC {t_oc:.2f}s; "
-                       f"extrapolated with GPU mean {pcg_iters:.0f} CG iters x {n_lc} LC and "
-                       f"{lob_iters:.0f} LOBPCG iters per SIMP iteration"),
-            "seconds_per_iter": round(per_iter, 3)}
+    n_lc = len(wl.loads)
+    per_iter = scale * (2 * t_asm + sum(pcg_iters) * t_cg + lob_iters * t_lob + n_lc * t_sens
+                        + 4 * t_filter + t_oc)
+    return {"value": round(1.0 / per_iter, 8), "unit": "SIMP iters/s", "cores": cores, "kind": "port",
+            "sample": (f"SciPy oracle (reference algorithm: Jacobi CG, Jacobi LOBPCG) on a "
+                       f"{sg.nx}x{sg.ny}x{sg.nz} z-slab of the {wl.name} grid ({sg.n_dofs} DOF; "
+                       f"times x{scale:.2f} for {geom.n_dofs} DOF): COO->CSR assembly {t_asm:.2f}s, "
+                       f"{n_cg} CG iters ({t_cg * 1e3:.1f} ms/iter), {n_lob} LOBPCG iters "
+                       f"({t_lob * 1e3:.1f} ms/iter), einsum sensitivities {t_sens:.2f}s, ndimage filter "
+                       f"{t_filter:.2f}s x4, OC {t_oc:.2f}s; per SIMP iteration "
+                       f"{sum(pcg_iters)} CG iters over {n_lc} load cases and {lob_iters} LOBPCG iters "
+                       f"({basis})"),
+            "seconds_per_iter": round(per_iter, 2)}
+
+
+def run_workload(name, precision, steps, warmup):
+    import numpy as np
+    wl, geom, fem, opt, F, x0 = make_optimizer(name, precision)
+    print(f"[bench] {wl.name}: {geom.nx}x{geom.ny}x{geom.nz}, {geom.n_dofs} DOF, {F.shape[0]} LC, "
+          f"{wl.n_modes} modes ({precision}); warmup {warmup}", file=sys.stderr, flush=True)
+    dt = time_steps(opt, warmup, steps)
+    h = opt.history
+    pcg_it = float(np.mean([np.mean(v) for v in h["pcg_iters"][-steps:]]))
+    lob_it = float(np.mean(h["lobpcg_iters"][-steps:]))
+    print(f"[bench] {wl.name}: {steps} steps in {dt:.3f}s; pcg {pcg_it:.0f} it/LC, lobpcg {lob_it:.0f} it",
+          file=sys.stderr, flush=True)
+    return wl, geom, fem, opt, dt, pcg_it, lob_it
 
 
 def main():
@@ -171,7 +232,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c5t")
+    ap.add_argument("--also", default="c2", help="comma-separated secondary workloads ('' for none)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-maxiter", type=int, default=2000,
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
@@ -182,20 +244,13 @@ def main():
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
     torch.cuda.set_device(0)
 
-    wl, geom, fem, opt, F, x0 = make_optimizer(args.workload, args.precision)
-    print(f"[bench] {wl.name}: {geom.nx}x{geom.ny}x{geom.nz}, {geom.n_dofs} DOF, "
-          f"{F.shape[0]} LC, {wl.n_modes} modes; warmup {args.warmup}", file=sys.stderr, flush=True)
-    dt = time_steps(opt, args.warmup, args.steps)
-    h = opt.history
-    pcg_it = float(np.mean([np.mean(v) for v in h["pcg_iters"][-args.steps:]]))
-    lob_it = float(np.mean(h["lobpcg_iters"][-args.steps:]))
-    print(f"[bench] timed {args.steps} steps in {dt:.3f}s; pcg {pcg_it:.0f} it, lobpcg {lob_it:.0f} it",
-          file=sys.stderr, flush=True)
+    wl, geom, fem, opt, dt, pcg_it, lob_it = run_workload(args.workload, args.precision, args.steps,
+                                                          args.warmup)
     value = args.steps / dt
+    roofs = kernel_rooflines(fem, geom, opt.rho_phys)
     plain = None
     if not args.no_plain and args.precision != "fp64":
         _, _, _, popt, _, _ = make_optimizer(args.workload, "fp64", modal_maxiter=args.plain_maxiter)
@@ -204,26 +259,44 @@ def main():
         plain = {"value": round(1 / pdt, 4), "unit": "SIMP iters/s", "ms_per_step": round(pdt * 1e3, 2),
                  "steps": 1, "path": "FP64 Jacobi-PCG + Jacobi-LOBPCG (reference preconditioner)",
                  "pcg_iters": ph["pcg_iters"][-1], "lobpcg_iters": ph["lobpcg_iters"][-1],
-                 "lobpcg_maxiter": args.plain_maxiter}
+                 "lobpcg_maxiter": args.plain_maxiter, "speedup_of_value": round(value * pdt, 2)}
         print(f"[bench] plain FP64 path: {pdt:.3f}s per SIMP iteration", file=sys.stderr, flush=True)
-    roof = stencil_roofline(fem, geom)
+        del popt
+    secondary = {}
+    for name in [s for s in args.also.split(",") if s and s != args.workload]:
+        swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, args.steps, args.warmup)
+        secondary[name] = {"value": round(args.steps / sdt, 4), "unit": "SIMP iters/s",
+                           "ms_per_step": round(sdt / args.steps * 1e3, 2), "n_dofs": sgeom.n_dofs,
+                           "load_cases": len(swl.loads), "modes": swl.n_modes,
+                           "mean_pcg_iters": round(spcg, 1), "mean_lobpcg_iters": round(slob, 1)}
+        del sopt
     line = {
         "metric": "SIMP iters/s (solve+6 modes+adjoint) & matvec HBM GB/s vs plain FP64 path",
         "value": round(value, 4), "unit": "SIMP iters/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-stencil/f64-refinement",
+        "dtype": "f64" if args.precision == "fp64" else "f64 Krylov / f32 multigrid",
         "data": "synthetic seeded voxel cantilever",
-        "config": {"workload": f"{wl.name} SIMP iteration {geom.nx}x{geom.ny}x{geom.nz} @ {geom.dx*1e3:g} mm, "
-                               f"{F.shape[0]} load case(s), {wl.n_modes} modes",
+        "config": {"workload": f"{wl.name}: SIMP iteration on {geom.nx}x{geom.ny}x{geom.nz} voxels @ "
+                               f"{geom.dx * 1e3:g} mm, {len(wl.loads)} load cases, {wl.n_modes} modes",
                    "n_dofs": geom.n_dofs, "pcg_rtol": wl.pcg_rtol, "lobpcg_tol": wl.modal_tol,
-                   "mean_pcg_iters": round(pcg_it, 1), "mean_lobpcg_iters": round(lob_it, 1),
+                   "mean_pcg_iters_per_lc": round(pcg_it, 1), "mean_lobpcg_iters": round(lob_it, 1),
                    "precision": args.precision},
-        "roofline": roof,
+        "roofline": roofs[DOMINANT],
+        "kernels": {k: v for k, v in roofs.items() if k != DOMINANT},
         "plain_fp64": plain,
+        "secondary": secondary,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(wl, geom, F, x0, pcg_it, lob_it)
+        if plain is not None:
+            pi = plain["pcg_iters"]
+            li = plain["lobpcg_iters"]
+            basis = f"iteration counts of the plain FP64 GPU path, LOBPCG capped at {args.plain_maxiter}"
+        else:
+            pi = [int(round(pcg_it))] * len(wl.loads)
+            li = int(round(lob_it))
+            basis = "iteration counts of the multigrid path (plain path not run)"
+        line["cpu_baseline"] = cpu_baseline(wl, geom, pi, li, basis)
     print(json.dumps(line), flush=True)
 
 

@@ -107,6 +107,15 @@ int vox8_lobpcg(vox8_ctx* ctx, int k, double tol, int maxiter, int prec, const d
  * may alias.  No reference counterpart: the reference preconditions with the
  * Jacobi diagonal (src/fem.py:441-443, 543-547). */
 int vox8_precond_apply(vox8_ctx* ctx, const double* d_r, double* d_z, int nrhs);
+/* One FP32 Chebyshev smoothing step on the fine grid, the building block of
+ * the multigrid preconditioner: d_xs += d_d; d_r -= K d_d;
+ * d_dnew = c1 d_d + c2 D^-1 d_r (D = diag K), nrhs column-major vectors
+ * (ld = n_dofs).  No reference counterpart (the reference preconditions with
+ * the Jacobi diagonal only, src/fem.py:543-547); exposed for custom smoothers
+ * and for timing the dominant kernel.  Needs the structured stencil. */
+int vox8_cheb_step(vox8_ctx* ctx, const float* d_d, float* d_r, float* d_xs, float* d_dnew,
+                   double c1, double c2, int nrhs);
+
 /* Level hierarchy: element counts per level (h_dims[3*l..3*l+2]) and the
  * smoother's lambda_max(D^-1 K) estimate per level. */
 int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, int max_levels);

@@ -69,3 +69,31 @@ def test_vcycle_contracts_energy(gpu):
         e = e - z.cpu().numpy()[0]
         en.append(e @ (K @ e))
     assert all(b < 0.7 * a for a, b in zip(en, en[1:])), en
+
+
+def test_cheb_step_matches_fp64_products(gpu):
+    """vox8_cheb_step (FP32, 3 columns) against FP64 K products and diagonal:
+    xs += d, r -= K d, d' = c1 d + c2 D^-1 r.  FP32 bar 1e-5 relative."""
+    from vox8 import _lib
+    geom, fem, rho, _ = _setup((70, 10, 9), 5, 0.1)
+    n = geom.n_dofs
+    rng = np.random.default_rng(4)
+    free = torch.zeros(n, dtype=torch.float64, device=gpu)
+    free[torch.as_tensor(fem.free_dofs, device=gpu)] = 1.0
+    d64 = torch.tensor(rng.standard_normal((3, n)), device=gpu) * free
+    r64 = torch.tensor(rng.standard_normal((3, n)), device=gpu) * free * 1e6
+    x64 = torch.tensor(rng.standard_normal((3, n)), device=gpu)
+    d, r, xs = d64.float().contiguous(), r64.float().contiguous(), x64.float().contiguous()
+    r0, xs0 = r.double(), xs.double()                  # inputs (r and xs are updated in place)
+    dn = torch.empty_like(d)
+    c1, c2 = 0.7, 3e-7
+    fem._call("vox8_cheb_step", _lib.ptr(d), _lib.ptr(r), _lib.ptr(xs), _lib.ptr(dn), c1, c2, 3)
+    Kd = fem.assemble_K(rho) @ d.double().T          # (n, 3), zero rows at fixed DOFs
+    diag = torch.as_tensor(fem.assemble_K(rho).diagonal(), device=gpu)
+    dinv = torch.where(free > 0, 1.0 / torch.where(diag != 0, diag, torch.ones_like(diag)),
+                       torch.zeros_like(diag))
+    r_ref = r0 - Kd.T
+    assert rel(xs.double().cpu().numpy(), (xs0 + d.double()).cpu().numpy()) < 1e-7
+    assert rel(r.double().cpu().numpy(), r_ref.cpu().numpy()) < 1e-5
+    dn_ref = c1 * d.double() + c2 * dinv * r_ref
+    assert rel(dn.double().cpu().numpy(), dn_ref.cpu().numpy()) < 1e-5

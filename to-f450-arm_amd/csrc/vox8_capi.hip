@@ -350,6 +350,32 @@ int vox8_precond_apply(vox8_ctx* c, const double* d_r, double* d_z, int nrhs) {
   API_END
 }
 
+int vox8_cheb_step(vox8_ctx* c, const float* d_d, float* d_r, float* d_xs, float* d_dnew, double c1,
+                   double c2, int nrhs) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_d && d_r && d_xs && d_dnew && nrhs >= 1 && nrhs <= 64, "bad argument");
+  VOX8_CHECK(d_dnew != d_d, "d_dnew must not alias d_d");
+  if (c->pattern == PAT_DENSE || c->stencil_kind != 0) {
+    set_error("vox8_cheb_step needs the structured stencil (box elements)");
+    return ERR_STATE;
+  }
+  VOX8_TRY(ensure_dinv(c, 0));
+  Epilogue<float> ep;
+  ep.mode = EPI_CHEB;
+  ep.r = d_r;
+  ep.xs = d_xs;
+  ep.dnew = d_dnew;
+  ep.dinv = c->dinvK32;
+  ep.c1 = (float)c1;
+  ep.c2 = (float)c2;
+  ep.ld = c->g.n_dofs;
+  return launch_apply_fast<float>(c->g, c->pattern, OP_K, c->c32, c->fK32, nullptr, c->d_fixed, d_d,
+                                  c->g.n_dofs, nullptr, nullptr, c->g.n_dofs, nrhs, c->st, DotFuse(),
+                                  ep);
+  API_END
+}
+
 int vox8_mg_info(vox8_ctx* c, int* h_nlevels, int* h_dims, double* h_lam_max, int max_levels) {
   API_BEGIN
   VOX8_TRY(require(c, true));

@@ -17,6 +17,7 @@ c2     220 x 32 x 24          1 mm      0.30 +- 0.05, cone r=1.5       LC1
 c3     220 x 32 x 24          1 mm      0.30 +- 0.05, cone r=1.5       LC1-5
 c4     nx 80..220 x 32 x 24   1 mm      V_f +- 0.25 V_f, cone r=1.5    LC1
 c5     440 x 64 x 48          0.5 mm    0.08 +- 0.02, cone r=1.5       LC1-5
+c5t    as c5, 6 modes instead of 12 (the 5-LC + 6-mode SIMP iteration)
 =====  =====================  ========  =============================  ======
 
 Material PA12-like: E 1.7 GPa, nu 0.4, rho 1010 kg/m^3, SIMP p = 3.
@@ -136,6 +137,8 @@ WORKLOADS = {
     "c2": Workload("c2", 220, 32, 24, 1e-3, 0.30, 0.05, True, ("LC1",), 6, 1e-8, 1e-6),
     "c3": Workload("c3", 220, 32, 24, 1e-3, 0.30, 0.05, True, ALL_LC, 6, 1e-8, 1e-6),
     "c5": Workload("c5", 440, 64, 48, 0.5e-3, 0.08, 0.02, True, ALL_LC, 12, 1e-10, 1e-6),
+    # the C5 grid with the 6-mode SIMP iteration (5 load cases + 6 modes + adjoint)
+    "c5t": Workload("c5t", 440, 64, 48, 0.5e-3, 0.08, 0.02, True, ALL_LC, 6, 1e-10, 1e-6),
 }
 
 
