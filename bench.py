@@ -239,6 +239,9 @@ def main():
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the per-kernel timings (and a 1 GiB device copy as a counter "
+                         "calibration), for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -246,6 +249,18 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+
+    if args.kernels_only:
+        wl, geom, fem, opt, F, x0 = make_optimizer(args.workload, args.precision)
+        src = torch.ones(1 << 27, dtype=torch.float64, device=fem.device)      # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        roofs = kernel_rooflines(fem, geom, torch.as_tensor(x0, device=fem.device), reps=5)
+        print(json.dumps({"workload": wl.name, "copy_bytes_each_way": src.numel() * 8,
+                          "kernels": roofs}), flush=True)
+        return
 
     wl, geom, fem, opt, dt, pcg_it, lob_it = run_workload(args.workload, args.precision, args.steps,
                                                           args.warmup)

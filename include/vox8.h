@@ -103,7 +103,7 @@ int vox8_lobpcg(vox8_ctx* ctx, int k, double tol, int maxiter, int prec, const d
                 uint64_t seed);
 
 /* The geometric-multigrid preconditioner used by the MIXED solvers:
- * z = V-cycle(r) for nrhs FP64 column-major vectors (ld = n_dofs); r and z
+ * z = cycle(r) (W-cycle by default) for nrhs FP64 column-major vectors (ld = n_dofs); r and z
  * may alias.  No reference counterpart: the reference preconditions with the
  * Jacobi diagonal (src/fem.py:441-443, 543-547). */
 int vox8_precond_apply(vox8_ctx* ctx, const double* d_r, double* d_z, int nrhs);
@@ -120,8 +120,12 @@ int vox8_cheb_step(vox8_ctx* ctx, const float* d_d, float* d_r, float* d_xs, flo
  * smoother's lambda_max(D^-1 K) estimate per level. */
 int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, int max_levels);
 /* Multigrid options: Chebyshev steps before/after (nu), smoothing interval
- * [lam_max/alpha, lam_max], coarsest-level size (DOFs) for the dense inverse. */
-int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs);
+ * [lam_max/alpha, lam_max], coarsest-level size (DOFs) for the dense inverse,
+ * cycle index (1 V-cycle, 2 W-cycle) applied on levels with at least
+ * w_min_dofs DOFs (smaller levels take one coarse correction).  Defaults:
+ * nu 2, alpha 12, coarse_dofs 700, cycle 2, w_min_dofs 50000. */
+int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, int cycle,
+                        int64_t w_min_dofs);
 
 /* Adjoint quantities for the current density:
  *   d_ce[e]  = sum_j w_j U_j,e^T Ke U_j,e      (elem_compliance, src/fem.py:605-615)

@@ -1,4 +1,6 @@
 // vox8 — C-ABI entry points (declared in include/vox8.h) and the context object.
+#include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -174,6 +176,8 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   if (w.kmax >= k) return OK;
   dfree(c, w.S); dfree(c, w.AS); dfree(c, w.BS); dfree(c, w.G); dfree(c, w.gram_partials);
   dfree(c, w.res); dfree(c, w.res_partials); dfree(c, w.counters); hfree(c, w.host);
+  dfree(c, w.guard);
+  w.guard_cols = 0;
   const int64_t n = c->g.n_dofs;
   VOX8_TRY(dalloc(c, &w.S, (size_t)3 * k * n));
   VOX8_TRY(dalloc(c, &w.AS, (size_t)3 * k * n));
@@ -189,6 +193,7 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   VOX8_TRY(dalloc(c, &w.counters, 4));
   VOX8_HIP(hipMemset(w.counters, 0, sizeof(unsigned int) * 4));
   VOX8_TRY(halloc(c, &w.host, 32));
+  VOX8_TRY(dalloc(c, &w.guard, (size_t)LOBPCG_GUARD_MAX * n));
   w.kmax = k;
   w.n = n;
   return OK;
@@ -268,6 +273,7 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   VOX8_CHECK((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "grid too large");
   vox8_ctx* c = new vox8_ctx();
   c->g = make_grid(nx, ny, nz);
+  if (const char* e = getenv("VOX8_MG_WMIN")) c->mg.w_min_dofs = atoll(e);   // tuning override
   c->st = (hipStream_t)hip_stream;
   int s = 0;
   auto fail = [&](int code) { vox8_destroy(c); *out = nullptr; return code; };
@@ -391,11 +397,16 @@ int vox8_mg_info(vox8_ctx* c, int* h_nlevels, int* h_dims, double* h_lam_max, in
   API_END
 }
 
-int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs) {
+int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs, int cycle,
+                        int64_t w_min_dofs) {
   VOX8_CHECK(c != nullptr, "null context");
-  VOX8_CHECK(nu >= 1 && nu <= 16 && alpha > 1.0 && coarse_dofs >= 24 && coarse_dofs <= 1000,
-             "mg options out of range (nu 1..16, alpha > 1, coarse_dofs 24..1000)");
+  VOX8_CHECK(nu >= 1 && nu <= 16 && alpha > 1.0 && coarse_dofs >= 24 && coarse_dofs <= 1000 &&
+                 cycle >= 1 && cycle <= 2 && w_min_dofs >= 0,
+             "mg options out of range (nu 1..16, alpha > 1, coarse_dofs 24..1000, cycle 1 or 2, "
+             "w_min_dofs >= 0)");
   c->mg.nu = nu;
+  c->mg.gamma = cycle;
+  c->mg.w_min_dofs = w_min_dofs;
   c->mg.alpha = alpha;
   if (c->mg.coarse_dofs != coarse_dofs) c->mg.built = false;
   c->mg.coarse_dofs = coarse_dofs;
@@ -413,6 +424,7 @@ int vox8_set_topology(vox8_ctx* c, const uint8_t* h_active, const uint8_t* h_fix
   c->h_active.assign(h_active, h_active + c->g.n_elem);
   c->h_fixed.assign(h_fixed, h_fixed + c->g.n_nodes);
   c->mg.built = false;             // level hierarchy follows the topology
+  c->lw.guard_cols = 0;            // guard columns of an earlier topology are not reused
   c->have_topo = true;
   c->have_density = false;
   c->dinv_valid = false;
@@ -533,7 +545,15 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
   VOX8_CHECK(tol > 0.0 && maxiter >= 0, "tol must be > 0");
   VOX8_CHECK(prec == VOX8_PREC_FP64 || prec == VOX8_PREC_MIXED, "prec must be FP64 or MIXED");
   VOX8_TRY(ensure_dinv(c, 1));
-  VOX8_TRY(ensure_lobpcg_work(c, k));
+  // optional guard columns (iterated, not returned; tuning switch
+  // VOX8_LOBPCG_GUARD): fewer iterations when lambda_k+1 is close to lambda_k,
+  // at a higher cost per iteration; a warm start reuses the last solve's
+  int guard = 0;
+  if (const char* e = getenv("VOX8_LOBPCG_GUARD")) guard = atoi(e);
+  guard = std::max(0, std::min(std::min(guard, LOBPCG_GUARD_MAX), 12 - k));
+  VOX8_TRY(ensure_lobpcg_work(c, k + guard));
+  if (!d_X0 || c->lw.guard_k != k) c->lw.guard_cols = 0;   // cold start: random guard columns
+  c->lw.guard_k = k;
   PcgOps<double> A;
   A.g = &c->g;
   A.fixed = c->d_fixed;
@@ -543,13 +563,13 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
   };
   LobpcgResult r{};
   if (prec == VOX8_PREC_MIXED) {
-    VOX8_TRY(ensure_mg(c, k));
+    VOX8_TRY(ensure_mg(c, k + guard));
     Precond<double> M = [c](const double* rr, double* z, int nr, hipStream_t st) {
       return mg_apply(c->mg, rr, z, nr, st);
     };
-    VOX8_TRY(lobpcg_fp64(A, km, k, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed, &M));
+    VOX8_TRY(lobpcg_fp64(A, km, k, guard, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed, &M));
   } else {
-    VOX8_TRY(lobpcg_fp64(A, km, k, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed));
+    VOX8_TRY(lobpcg_fp64(A, km, k, guard, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed));
   }
   if (h_iters) *h_iters = r.iters;
   if (h_info) *h_info = r.info;

@@ -41,7 +41,9 @@ constexpr int CUBE_CLASS[45] = {0, 1, 1, 2, 2, 2, 2, 2, 2, 1, 0, 1, 2, 2, 2, 2, 
 int fast_coefficients(const double* Ke, const double* Me, FastCoef<double>* c64,
                       FastCoef<float>* c32);
 
-enum EpiMode : int { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB = 2 };
+// EPI_CHEB0: the first Chebyshev step from a zero iterate, x = 0, r = b:
+// the input vector is b and the direction d = c0 D^-1 b is formed on load.
+enum EpiMode : int { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB = 2, EPI_CHEB0 = 3 };
 
 // Node-local epilogue of a K product (multigrid smoothing), vectors with ld.
 template <typename T>
@@ -52,7 +54,7 @@ struct Epilogue {
   T* xs = nullptr;           // CHEB: iterate (+= d)
   T* dnew = nullptr;         // out: next direction
   const T* dinv = nullptr;   // Jacobi inverse diagonal (one vector, no ld)
-  T c1 = 0, c2 = 0;
+  T c0 = 0, c1 = 0, c2 = 0;
   int64_t ld = 0;
 };
 

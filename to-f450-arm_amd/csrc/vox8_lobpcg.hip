@@ -255,28 +255,27 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
 //   D = diag(GA)^-1/2, GA_s = D GA D = V diag(s) V^T, keep s > drop * max(s),
 //   Q = V_keep diag(s^-1/2), H = Q^T (D GB D) Q = Y diag(mu) Y^T,
 //   lam = 1/mu for the k largest mu, c = D Q y / sqrt(mu)  (M-normalised).
-// Out (global): C (m x k, row-major), lam (k), status (0 ok, 1 failure).
-__global__ void __launch_bounds__(RR_THREADS)
-k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__ C,
-                double* __restrict__ lam, double* __restrict__ status, double drop_tol,
-                double jeps) {
+// Works on the leading m x m block of G (row stride ld).  Returns true on
+// failure (fewer than k directions kept or a non-positive mu).
+__device__ bool rr_solve(int m, int ld, int k, const double* __restrict__ G, double* __restrict__ C,
+                         double* __restrict__ lam, double drop_tol, double jeps) {
   __shared__ double A[MMAX * MMAX], B[MMAX * MMAX], V[MMAX * MMAX], T1[MMAX * MMAX];
   __shared__ double dsc[MMAX], sig[MMAX], mu[MMAX];
   __shared__ int keep[MMAX], perm[MMAX];
-  __shared__ int s_r;
+  __shared__ int s_r, s_bad;
   const int tid = threadIdx.x, nth = blockDim.x;
   const double* GA = G;
-  const double* GB = G + m * m;
+  const double* GB = G + ld * ld;
   for (int i = tid; i < m; i += nth) {
-    const double d = GA[i * m + i];
+    const double d = GA[i * ld + i];
     dsc[i] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
   }
   __syncthreads();
   for (int e = tid; e < m * m; e += nth) {
     const int i = e / m, j = e % m;
     const double sc = dsc[i] * dsc[j];
-    A[i * MMAX + j] = 0.5 * (GA[i * m + j] + GA[j * m + i]) * sc;
-    B[i * MMAX + j] = 0.5 * (GB[i * m + j] + GB[j * m + i]) * sc;
+    A[i * MMAX + j] = 0.5 * (GA[i * ld + j] + GA[j * ld + i]) * sc;
+    B[i * MMAX + j] = 0.5 * (GB[i * ld + j] + GB[j * ld + i]) * sc;
   }
   __syncthreads();
   jacobi_eig(A, V, m, jeps);
@@ -325,20 +324,44 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
     }
     bool bad = r < k;
     for (int j = 0; j < k && j < r; ++j) bad = bad || !(mu[perm[j]] > 0.0);
-    status[0] = bad ? 1.0 : 0.0;
+    s_bad = bad;
   }
   __syncthreads();
-  const int kk = r < k ? r : k;
-  for (int e = tid; e < m * k; e += nth) {
-    const int i = e / k, j = e % k;
-    double s = 0.0;
-    if (j < kk && mu[perm[j]] > 0.0) {
+  const bool bad = s_bad;
+  if (!bad) {
+    for (int e = tid; e < m * k; e += nth) {
+      const int i = e / k, j = e % k;
+      double s = 0.0;
       for (int l = 0; l < r; ++l) s += T1[i * MMAX + l] * V[l * MMAX + perm[j]];
-      s *= dsc[i] / sqrt(mu[perm[j]]);
+      C[i * k + j] = s * dsc[i] / sqrt(mu[perm[j]]);
     }
-    C[i * k + j] = s;
+    for (int j = tid; j < k; j += nth) lam[j] = 1.0 / mu[perm[j]];
   }
-  for (int j = tid; j < k; j += nth) lam[j] = (j < kk && mu[perm[j]] > 0.0) ? 1.0 / mu[perm[j]] : 0.0;
+  __syncthreads();
+  return bad;
+}
+
+// Rayleigh-Ritz on [X | W | P] (m = 3k) with the restart of LOBPCG done on
+// the device: if the full basis fails, [X | W] is used and the P rows of C are
+// zero (so the update is X' = X C_X + W C_W, P' = W C_W); if that fails too,
+// C keeps X unchanged and status = 1.
+// Out (global): C (m x k, row-major), lam (k), status (0 ok, 1 failure).
+__global__ void __launch_bounds__(RR_THREADS)
+k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__ C,
+                double* __restrict__ lam, double* __restrict__ status, double drop_tol,
+                double jeps) {
+  bool bad = rr_solve(m, m, k, G, C, lam, drop_tol, jeps);
+  int used = m;
+  if (bad && m > 2 * k) {
+    bad = rr_solve(2 * k, m, k, G, C, lam, drop_tol, jeps);
+    used = 2 * k;
+  }
+  if (bad) used = 0;
+  for (int e = threadIdx.x + used * k; e < m * k; e += blockDim.x) C[e] = 0.0;
+  __syncthreads();
+  if (bad)
+    for (int j = threadIdx.x; j < k; j += blockDim.x) C[j * k + j] = 1.0;
+  if (threadIdx.x == 0) status[0] = bad ? 1.0 : 0.0;
 }
 
 // ---- Ritz update (templated on the block size) ------------------------------
@@ -526,10 +549,12 @@ static int read_state(LobpcgWork& w, int k, hipStream_t st) {
 
 int lobpcg_fp64(const PcgOps<double>& A,
                 const std::function<int(const double*, double*, double*, int, hipStream_t)>& applyKM,
-                int k, double tol, int maxiter, const double* X0, double* Xout, double* lam_out,
-                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
+                int kw, int guard, double tol, int maxiter, const double* X0, double* Xout,
+                double* lam_out, LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
                 const Precond<double>* M) {
-  VOX8_CHECK(k >= 1 && k <= 12, "lobpcg: block size must be 1..12");
+  const int k = kw + guard;                     // iterated block: wanted pairs + guard columns
+  VOX8_CHECK(kw >= 1 && guard >= 0 && guard <= LOBPCG_GUARD_MAX && k <= 12 && k <= w.kmax,
+             "lobpcg: block size must be 1..12");
   const int64_t n = w.n;
   double* X = w.S;
   double* W = w.S + (int64_t)k * n;
@@ -540,8 +565,18 @@ int lobpcg_fp64(const PcgOps<double>& A,
   double drop_tol = 1e-13;
   if (const char* e = getenv("VOX8_LOBPCG_DROP")) drop_tol = atof(e);
   if (X0) {
-    hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * k)), dim3(256), 0, st, n, k, X0, X);
+    hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * kw)), dim3(256), 0, st, n, kw, X0, X);
     VOX8_HIP(hipGetLastError());
+    if (guard > 0) {
+      // guard columns: the previous solve's, else random
+      if (w.guard_cols >= guard)
+        hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * guard)), dim3(256), 0, st, n, guard,
+                           (const double*)w.guard, X + (int64_t)kw * n);
+      else
+        hipLaunchKernelGGL(k_random_block, dim3(grid_blocks(n)), dim3(256), 0, st, n, guard,
+                           seed ^ 0x5bd1e995ull, A.fixed, X + (int64_t)kw * n);
+      VOX8_HIP(hipGetLastError());
+    }
   } else {
     hipLaunchKernelGGL(k_random_block, dim3(grid_blocks(n)), dim3(256), 0, st, n, k, seed, A.fixed, X);
     VOX8_HIP(hipGetLastError());
@@ -554,10 +589,10 @@ int lobpcg_fp64(const PcgOps<double>& A,
   if (M) VOX8_TRY((*M)(W, W, k, st));
   VOX8_TRY(read_state(w, k, st));
   VOX8_CHECK(w.host[12] == 0.0, "lobpcg: linearly dependent initial block");
-  std::vector<int> conv(k, 0);
+  std::vector<int> conv(kw, 0);
   auto update_conv = [&]() {
     bool all = true;
-    for (int j = 0; j < k; ++j) {
+    for (int j = 0; j < kw; ++j) {
       if (sqrt(fabs(w.host[13 + j])) < tol) conv[j] = 1;
       all = all && conv[j];
     }
@@ -568,20 +603,15 @@ int lobpcg_fp64(const PcgOps<double>& A,
   bool have_p = false;
   while (!all && it < maxiter) {
     VOX8_TRY(applyKM(W, AW, BW, k, st));
-    int m = have_p ? 3 * k : 2 * k;
+    const int m = have_p ? 3 * k : 2 * k;
+    // the Rayleigh-Ritz kernel restarts without P on the device when needed
     VOX8_TRY(gram_and_rr(w, m, k, st, drop_tol));
-    VOX8_TRY(read_state(w, k, st));
-    if (w.host[12] != 0.0 && have_p) {     // rank-deficient basis: restart without P
-      m = 2 * k;
-      VOX8_TRY(gram_and_rr(w, m, k, st, drop_tol));
-      VOX8_TRY(read_state(w, k, st));
-    }
-    if (w.host[12] != 0.0) break;           // cannot continue: return the current block
     VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st));
     if (M) VOX8_TRY((*M)(W, W, k, st));
     have_p = true;
     ++it;
     VOX8_TRY(read_state(w, k, st));
+    if (w.host[12] != 0.0) break;           // basis collapsed: X was kept, stop here
     all = update_conv();
   }
   // final Rayleigh-Ritz on X alone (scipy lobpcg post-processing); AX, BX
@@ -589,11 +619,17 @@ int lobpcg_fp64(const PcgOps<double>& A,
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
   VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st));
   VOX8_TRY(read_state(w, k, st));
-  hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * k)), dim3(256), 0, st, n, k,
+  hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * kw)), dim3(256), 0, st, n, kw,
                      (const double*)X, Xout);
   VOX8_HIP(hipGetLastError());
+  if (guard > 0) {
+    hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * guard)), dim3(256), 0, st, n, guard,
+                       (const double*)(X + (int64_t)kw * n), w.guard);
+    VOX8_HIP(hipGetLastError());
+    w.guard_cols = guard;
+  }
   double mx = 0.0;
-  for (int j = 0; j < k; ++j) {
+  for (int j = 0; j < kw; ++j) {
     lam_out[j] = w.host[j];
     mx = fmax(mx, sqrt(fabs(w.host[13 + j])));
   }

@@ -28,6 +28,9 @@ struct MultiGrid {
   std::vector<MGLevel> lv;
   int ncols = 0;               // capacity of the work vectors
   int nu = 2;                  // Chebyshev smoothing steps before and after
+  int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
+  int64_t w_min_dofs = 50000;  // levels smaller than this take one correction (V) even when
+                               // gamma = 2: their kernels are launch-latency bound
   int coarse_dofs = 700;       // stop coarsening at this many DOFs (dense inverse)
   double alpha = 12.0;         // smoothing interval [lam_max/alpha, lam_max]
   // coarsest level: dense inverse when small
@@ -58,7 +61,8 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
 int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t st);
 // Ensure work vectors for ncols columns.
 int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al);
-// z = V-cycle(r) for ncols FP64 column-major vectors (ld = fine n_dofs).
+// z = cycle(r) (V or W, see MultiGrid::gamma) for ncols FP64 column-major
+// vectors (ld = fine n_dofs).
 int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st);
 
 }  // namespace vox8

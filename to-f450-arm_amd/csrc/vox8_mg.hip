@@ -70,19 +70,6 @@ __global__ void k_mg_jacobi(Grid g, const double* __restrict__ f, Diag24 dg,
     dinv[3 * n + c] = ((fx >> c) & 1) ? 0.0f : (d[c] != 0.0 ? (float)(1.0 / d[c]) : 1.0f);
 }
 
-// r = b, d = s * dinv * b, x = 0
-__global__ void k_mg_cheb_init(int64_t n, const float* __restrict__ b, const float* __restrict__ dinv,
-                               float s, float* __restrict__ r, float* __restrict__ d,
-                               float* __restrict__ x) {
-  const int64_t off = (int64_t)blockIdx.y * n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const float v = b[off + i];
-    r[off + i] = v;
-    d[off + i] = s * dinv[i] * v;
-    x[off + i] = 0.0f;
-  }
-}
 
 // coarse b = R r: weights 1 at the coincident fine node, 1/2 at the neighbours, per axis
 __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
@@ -487,66 +474,65 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
 
 struct ChebCoef { double theta, delta, sigma; };
 
-static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st) {
+// nu Chebyshev steps (three-term recurrence).  from_zero: the iterate starts
+// at x = 0, r = b and the first step (EPI_CHEB0) reads b and forms the first
+// direction itself; otherwise the direction left in L.d by a residual pass is
+// continued from the current x, r.
+static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st, bool from_zero) {
   const double lmax = L.lam_max, lmin = lmax / alpha;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
   double rho = 1.0 / sigma;
-  float* d = L.d;
-  float* d2 = L.d2;
+  const float* in = from_zero ? L.b : L.d;
+  float* out = from_zero ? L.d : L.d2;
   for (int k = 0; k < nu; ++k) {
     const double rho1 = 1.0 / (2.0 * sigma - rho);
     Epilogue<float> ep;
-    ep.mode = EPI_CHEB;
+    ep.mode = (k == 0 && from_zero) ? EPI_CHEB0 : EPI_CHEB;
     ep.r = L.r;
     ep.xs = L.x;
-    ep.dnew = d2;
+    ep.dnew = out;
     ep.dinv = L.dinv;
+    ep.c0 = (float)(1.0 / theta);
     ep.c1 = (float)(rho1 * rho);
     ep.c2 = (float)(2.0 * rho1 / delta);
     ep.ld = L.g.n_dofs;
-    VOX8_TRY(level_apply(L, d, nullptr, ncols, st, ep));
-    std::swap(d, d2);
+    VOX8_TRY(level_apply(L, in, nullptr, ncols, st, ep));
+    in = out;
+    out = (out == L.d) ? L.d2 : L.d;
     rho = rho1;
   }
   return OK;
 }
 
-int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st) {
-  VOX8_CHECK(mg.updated && ncols <= mg.ncols, "multigrid not set up for this density / width");
+// Cycle on level l: L.x = MG_l(L.b).  Pre-smoothing from zero, gamma
+// coarse-grid corrections (each: restrict the residual, cycle on l+1, add the
+// prolonged correction, recompute the residual and the first Chebyshev
+// direction), post-smoothing.  Levels below w_min_dofs take one correction.  The coarsest level is the dense inverse, or
+// 8 nu smoothing steps from zero when it is too large for one.  With the same
+// polynomial before and after and symmetric transfers the operator is
+// symmetric, as PCG requires.
+static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st) {
   const int nl = (int)mg.lv.size();
-  MGLevel& F = mg.lv[0];
-  hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b);
-  VOX8_HIP(hipGetLastError());
-  const int last_smoothed = mg.n_coarse ? nl - 2 : nl - 1;
-  auto theta_of = [&](const MGLevel& L) { return 0.5 * (L.lam_max + L.lam_max / mg.alpha); };
-  // descending leg
-  for (int l = 0; l <= last_smoothed; ++l) {
-    MGLevel& L = mg.lv[l];
-    const int64_t n = L.g.n_dofs;
-    hipLaunchKernelGGL(k_mg_cheb_init, dim3(nb(n), ncols), dim3(256), 0, st, n, (const float*)L.b,
-                       (const float*)L.dinv, (float)(1.0 / theta_of(L)), L.r, L.d, L.x);
-    VOX8_HIP(hipGetLastError());
-    const int nu = (l == nl - 1) ? 8 * mg.nu : mg.nu;   // no coarse grid below: smooth harder
-    VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st));
-    if (l + 1 < nl) {
-      MGLevel& Cn = mg.lv[l + 1];
-      hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
-                         L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b);
-      VOX8_HIP(hipGetLastError());
-    }
-  }
-  if (mg.n_coarse) {
-    MGLevel& C = mg.lv.back();
+  MGLevel& L = mg.lv[l];
+  if (l == nl - 1 && mg.n_coarse) {
     hipLaunchKernelGGL(k_mg_coarse_apply, dim3((mg.n_coarse + 15) / 16, ncols), dim3(256), 0, st,
-                       mg.n_coarse, (const float*)mg.Ainv, (const float*)C.b, C.x);
+                       mg.n_coarse, (const float*)mg.Ainv, (const float*)L.b, L.x);
     VOX8_HIP(hipGetLastError());
+    return OK;
   }
-  // ascending leg
-  for (int l = nl - 2; l >= 0; --l) {
-    MGLevel& L = mg.lv[l];
-    const MGLevel& Cn = mg.lv[l + 1];
-    hipLaunchKernelGGL(k_mg_prolong_add, dim3((unsigned)((L.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
-                       Cn.g, L.g, (const float*)Cn.x, (const uint8_t*)L.fixed, L.x);
+  const double theta = 0.5 * (L.lam_max + L.lam_max / mg.alpha);
+  if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true);   // no coarse grid below
+  VOX8_TRY(smooth(L, mg.nu, mg.alpha, ncols, st, true));
+  MGLevel& Cn = mg.lv[l + 1];
+  const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
+  const int g = (exact_below || L.g.n_dofs < mg.w_min_dofs) ? 1 : mg.gamma;
+  for (int j = 0; j < g; ++j) {
+    hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
+                       L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b);
+    VOX8_HIP(hipGetLastError());
+    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st));
+    hipLaunchKernelGGL(k_mg_prolong_add, dim3((unsigned)((L.g.n_nodes + 255) / 256), ncols), dim3(256), 0,
+                       st, Cn.g, L.g, (const float*)Cn.x, (const uint8_t*)L.fixed, L.x);
     VOX8_HIP(hipGetLastError());
     Epilogue<float> ep;
     ep.mode = EPI_RESID;
@@ -554,11 +540,19 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
     ep.r = L.r;
     ep.dnew = L.d;
     ep.dinv = L.dinv;
-    ep.c2 = (float)(1.0 / theta_of(L));
+    ep.c2 = (float)(1.0 / theta);
     ep.ld = L.g.n_dofs;
     VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
-    VOX8_TRY(smooth(L, mg.nu, mg.alpha, ncols, st));
   }
+  return smooth(L, mg.nu, mg.alpha, ncols, st, false);
+}
+
+int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st) {
+  VOX8_CHECK(mg.updated && ncols <= mg.ncols, "multigrid not set up for this density / width");
+  MGLevel& F = mg.lv[0];
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b);
+  VOX8_HIP(hipGetLastError());
+  VOX8_TRY(mg_cycle(mg, 0, ncols, st));
   hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
                      (const float*)F.x, z);
   VOX8_HIP(hipGetLastError());

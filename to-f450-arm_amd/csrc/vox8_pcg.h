@@ -67,15 +67,22 @@ struct LobpcgWork {
   double* host = nullptr;     // pinned mirror of lam|res|status
   double* res_partials = nullptr;
   unsigned int* counters = nullptr;
+  double* guard = nullptr;    // last solve's guard vectors [gmax][n] (warm start of the next)
+  int guard_cols = 0;         // columns held in `guard` (0 = none)
+  int guard_k = 0;            // wanted pairs of the solve that left them
   int kmax = 0;
   int64_t n = 0;
   int gram_chunks = 0;
 };
 
+constexpr int LOBPCG_GUARD_MAX = 2;
+
+// k wanted pairs, iterated with `guard` extra columns (k + guard <= 12); only
+// the k lowest are tested for convergence and returned.
 int lobpcg_fp64(const PcgOps<double>& A, const std::function<int(const double*, double*, double*,
                                                                  int, hipStream_t)>& applyKM,
-                int k, double tol, int maxiter, const double* X0, double* Xout, double* lam_out,
-                LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
+                int k, int guard, double tol, int maxiter, const double* X0, double* Xout,
+                double* lam_out, LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
                 const Precond<double>* M = nullptr);
 
 }  // namespace vox8

@@ -100,6 +100,8 @@ __global__ void k_pcg_dir(int64_t n_dofs, const T* __restrict__ dinv, const T* _
 }
 
 // End of an iteration: alpha = rho/(p.q); x += alpha p; r -= alpha q; new rho, ||r||^2.
+// The grid-stride loop is unrolled 4x (16 independent loads in flight per
+// thread); each thread's partial sums keep a fixed order.
 template <typename T, bool RZ>
 __global__ void __launch_bounds__(256)
 k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
@@ -112,12 +114,31 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
   if (s.done || s.iters >= maxiter) return;
   const double alpha = s.rz / s.pq;
   double rr = 0.0, rz = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dofs;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = col * ld + i;
-    x[j] = (T)((double)x[j] + alpha * (double)p[j]);
-    const T rn = (T)((double)r[j] - alpha * (double)q[j]);
-    r[j] = rn;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  x += col * ld; r += col * ld; p += col * ld; q += col * ld;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n_dofs; i += 4 * stride) {
+    T xv[4], pv[4], rv[4], qv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t j = i + u * stride;
+      xv[u] = x[j]; pv[u] = p[j]; rv[u] = r[j]; qv[u] = q[j];
+      if (RZ) dv[u] = dinv[j];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t j = i + u * stride;
+      x[j] = (T)((double)xv[u] + alpha * (double)pv[u]);
+      const T rn = (T)((double)rv[u] - alpha * (double)qv[u]);
+      r[j] = rn;
+      rr += (double)rn * (double)rn;
+      if (RZ) rz += (double)rn * (double)dv[u] * (double)rn;
+    }
+  }
+  for (; i < n_dofs; i += stride) {
+    x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+    const T rn = (T)((double)r[i] - alpha * (double)q[i]);
+    r[i] = rn;
     rr += (double)rn * (double)rn;
     if (RZ) rz += (double)rn * (double)dinv[i] * (double)rn;
   }
@@ -152,9 +173,17 @@ k_pcg_rz(int64_t n_dofs, const T* __restrict__ r, const T* __restrict__ z, int64
   const PcgScalars s = scal[col];
   if (s.done || s.iters >= maxiter) return;
   double rz = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dofs;
-       i += (int64_t)gridDim.x * blockDim.x)
-    rz += (double)r[col * ld + i] * (double)z[col * ld + i];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  r += col * ld; z += col * ld;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n_dofs; i += 4 * stride) {
+    T rv[4], zv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { rv[u] = r[i + u * stride]; zv[u] = z[i + u * stride]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rz += (double)rv[u] * (double)zv[u];
+  }
+  for (; i < n_dofs; i += stride) rz += (double)r[i] * (double)z[i];
   double v = block_sum(rz, sh), tot;
   if (grid_reduce_last<1>(&v, partials + (int64_t)col * gridDim.x, counters + col, gridDim.x,
                           blockIdx.x, &tot, sh)) {
@@ -166,17 +195,25 @@ k_pcg_rz(int64_t n_dofs, const T* __restrict__ r, const T* __restrict__ z, int64
 }
 
 template <typename T>
-__global__ void k_pcg_dir_z(int64_t n_dofs, const T* __restrict__ z, T* __restrict__ p, int64_t ld,
-                            const PcgScalars* __restrict__ scal, int maxiter) {
+__global__ void __launch_bounds__(256)
+k_pcg_dir_z(int64_t n_dofs, const T* __restrict__ z, T* __restrict__ p, int64_t ld,
+            const PcgScalars* __restrict__ scal, int maxiter) {
   const int col = blockIdx.y;
   const PcgScalars s = scal[col];
   if (s.done || s.iters >= maxiter) return;
   const double beta = s.iters > 0 ? s.rz / s.rz_prev : 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_dofs;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = col * ld + i;
-    p[j] = (T)(s.iters > 0 ? (double)z[j] + beta * (double)p[j] : (double)z[j]);
+  const bool first = s.iters == 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  z += col * ld; p += col * ld;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n_dofs; i += 4 * stride) {
+    T zv[4], pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { zv[u] = z[i + u * stride]; pv[u] = first ? T(0) : p[i + u * stride]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[i + u * stride] = (T)((double)zv[u] + beta * (double)pv[u]);
   }
+  for (; i < n_dofs; i += stride) p[i] = (T)(first ? (double)z[i] : (double)z[i] + beta * (double)p[i]);
 }
 
 // x64 += mask(d) (iterative-refinement correction), per column.

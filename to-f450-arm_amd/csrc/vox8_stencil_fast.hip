@@ -210,7 +210,9 @@ __device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mo
 
 // MODE (K only): EPI_PLAIN y = A x; EPI_RESID r = b - A x, d = c2 D^-1 r;
 // EPI_CHEB one Chebyshev smoothing step with x holding the direction d:
-//   xs += d, r -= A d, d' = c1 d + c2 D^-1 r   (all node-local once A d is known)
+//   xs += d, r -= A d, d' = c1 d + c2 D^-1 r   (all node-local once A d is known);
+// EPI_CHEB0 the same step from xs = 0, r = b with x holding b: d = c0 D^-1 b
+//   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r.
 template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE>
 __global__ void __launch_bounds__(NT)
 k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const T* __restrict__ fK,
@@ -230,7 +232,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
     ep.r += col * ep.ld;
     ep.dnew += col * ep.ld;
     if (MODE == EPI_RESID) ep.b += col * ep.ld;
-    if (MODE == EPI_CHEB) ep.xs += col * ep.ld;
+    if (MODE == EPI_CHEB || MODE == EPI_CHEB0) ep.xs += col * ep.ld;
   }
   // XCD-aware tile number: XCD j = bid % 8 gets the contiguous range
   // [j q + min(j, r), ...) of the nb = 8 q + r tiles
@@ -277,6 +279,11 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
         v0 = x[3u * n];
         v1 = x[3u * n + 1u];
         v2 = x[3u * n + 2u];
+        if (MODE == EPI_CHEB0) {
+          v0 *= ep.c0 * ep.dinv[3u * n];
+          v1 *= ep.c0 * ep.dinv[3u * n + 1u];
+          v2 *= ep.c0 * ep.dinv[3u * n + 2u];
+        }
         v0 = (f & 1u) ? T(0) : v0;
         v1 = (f & 2u) ? T(0) : v1;
         v2 = (f & 4u) ? T(0) : v2;
@@ -324,6 +331,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
       for (int c = 0; c < 3; ++c) {
         if (MODE == EPI_RESID) eb[c] = ep.b[3u * nn + c];
         if (MODE == EPI_CHEB) er[c] = ep.r[3u * nn + c];
+        if (MODE == EPI_CHEB0) er[c] = x[3u * nn + c];          // r = b
         if (MODE != EPI_PLAIN) ed[c] = ep.dinv[3u * nn + c];
       }
     }
@@ -388,9 +396,14 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
               const T r = fx ? T(0) : eb[c] - val;
               ep.r[3u * nn + c] = r;
               ep.dnew[3u * nn + c] = ep.c2 * ed[c] * r;
-            } else {
+            } else if (MODE == EPI_CHEB) {
               ep.xs[3u * nn + c] += xin[c];
               const T r = er[c] - val;
+              ep.r[3u * nn + c] = r;
+              ep.dnew[3u * nn + c] = ep.c1 * xin[c] + ep.c2 * ed[c] * r;
+            } else {                                            // EPI_CHEB0
+              ep.xs[3u * nn + c] = xin[c];
+              const T r = fx ? T(0) : er[c] - val;
               ep.r[3u * nn + c] = r;
               ep.dnew[3u * nn + c] = ep.c1 * xin[c] + ep.c2 * ed[c] * r;
             }
@@ -461,6 +474,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
   if (op == OP_K) {                                                                     \
     if (ep.mode == EPI_RESID) VOX8_LAUNCH(PAT, true, false, false, EPI_RESID);          \
     else if (ep.mode == EPI_CHEB) VOX8_LAUNCH(PAT, true, false, false, EPI_CHEB);       \
+    else if (ep.mode == EPI_CHEB0) VOX8_LAUNCH(PAT, true, false, false, EPI_CHEB0);     \
     else if (fz) VOX8_LAUNCH(PAT, true, false, true, EPI_PLAIN);                         \
     else VOX8_LAUNCH(PAT, true, false, false, EPI_PLAIN);                               \
   } else if (op == OP_M) VOX8_LAUNCH(PAT, false, true, false, EPI_PLAIN);                \
