@@ -17,7 +17,8 @@ Prints one JSON line.  Extra objects:
   kernels       the same measurement for the other hot kernels
   plain_fp64    one SIMP iteration of the same loop on the plain FP64 path
                 (the reference's Jacobi preconditioner in PCG and LOBPCG)
-  secondary     the other workloads given with --also (default c2)
+  secondary     the other workloads given with --also (default c2, c3 and c4,
+                the 16 batched designs run concurrently and one by one)
   cpu_baseline  the SciPy oracle on this host (bounded z-slab sample of the
                 same grid, scaled by DOF count and by the plain path's
                 iteration counts, i.e. the reference's algorithm)
@@ -227,13 +228,35 @@ def run_workload(name, precision, steps, warmup):
     return wl, geom, fem, opt, dt, pcg_it, lob_it
 
 
+def bench_c4(precision, steps, warmup):
+    """The 16 uneven designs: concurrent (one context, stream and thread per
+    design) against one after another; design-iterations per second."""
+    from vox8.batch import make_designs, step_designs
+    from vox8.workloads import c4_designs
+    import torch
+    out = {"unit": "design SIMP iters/s", "designs": 16}
+    for mode, conc in (("concurrent", True), ("sequential", False)):
+        ds = make_designs(c4_designs(), precision, concurrent=conc)
+        step_designs(ds, warmup, concurrent=conc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step_designs(ds, steps, concurrent=conc)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[mode] = round(16 * steps / dt, 3)
+        print(f"[bench] c4 {mode}: {16 * steps} design-iterations in {dt:.3f}s", file=sys.stderr, flush=True)
+        del ds
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c5t")
-    ap.add_argument("--also", default="c2", help="comma-separated secondary workloads ('' for none)")
+    ap.add_argument("--also", default="c2,c3,c4",
+                    help="comma-separated secondary workloads ('' for none; c4 = the 16 batched designs)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-maxiter", type=int, default=2000,
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
@@ -278,7 +301,11 @@ def main():
         print(f"[bench] plain FP64 path: {pdt:.3f}s per SIMP iteration", file=sys.stderr, flush=True)
         del popt
     secondary = {}
-    for name in [s for s in args.also.split(",") if s and s != args.workload]:
+    also = [s for s in args.also.split(",") if s and s != args.workload]
+    if "c4" in also:
+        also.remove("c4")
+        secondary["c4"] = bench_c4(args.precision, args.steps, args.warmup)
+    for name in also:
         swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, args.steps, args.warmup)
         secondary[name] = {"value": round(args.steps / sdt, 4), "unit": "SIMP iters/s",
                            "ms_per_step": round(sdt / args.steps * 1e3, 2), "n_dofs": sgeom.n_dofs,

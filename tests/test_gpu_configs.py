@@ -1,0 +1,60 @@
+"""GPU: the block multi-load-case configuration (c3) and the batched designs
+configuration (c4).
+
+c3: 220x32x24 at 1 mm, LC1-LC5 solved together as one block PCG (rtol 1e-8)
+against each load case solved alone: per-load-case compliance within 1e-6
+displacements within 1e-6, and the weighted summed
+compliance gradient equal to the weighted sum of the single-case gradients.
+
+c4: the 16 uneven designs (nx 80..220, V_f 0.08..0.50), two SIMP iterations
+each, run concurrently (one context, stream and thread per design) against
+the same designs run one after another: the histories must agree bitwise-close
+(deterministic kernels, independent state).
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c3_block_pcg_matches_single_solves(gpu):
+    from vox8.fem import FEM3D, Material
+    from vox8.workloads import E_PA, NU, RHO_MAT, WORKLOADS, load_cases, seeded_density
+    wl = WORKLOADS["c3"]
+    geom = wl.geometry()
+    fem = FEM3D(geom, Material(E=E_PA, nu=NU, rho=RHO_MAT), precision="mixed")
+    F = load_cases(geom, wl.loads)
+    assert F.shape == (5, geom.n_dofs)
+    rho = torch.tensor(seeded_density(geom, wl.base, wl.amp, wl.seed), device=gpu)
+    Ft = torch.tensor(F, device=gpu)
+    U, info = fem.solve_block(rho, Ft, rtol=wl.pcg_rtol)
+    assert info["info"] == [0] * 5
+    w = np.array([0.3, 0.2, 0.2, 0.2, 0.1])
+    s = fem.sensitivities(rho, U=U, weights=w)
+    dc_sum = torch.zeros_like(s["dc"])
+    for j in range(5):
+        Uj, ij = fem.solve_block(rho, Ft[j:j + 1], rtol=wl.pcg_rtol)
+        assert ij["info"] == [0]
+        cj = float(Ft[j] @ Uj[0])
+        assert abs(cj / float(Ft[j] @ U[j]) - 1) < 1e-6
+        assert abs(s["comp"][j] / cj - 1) < 1e-6
+        assert rel(U[j].cpu().numpy(), Uj[0].cpu().numpy()) < 1e-6
+        dc_sum += w[j] * fem.sensitivities(rho, U=Uj, weights=[1.0])["dc"]
+    assert rel(s["dc"].cpu().numpy(), dc_sum.cpu().numpy()) < 1e-6
+
+
+def test_c4_batched_designs_match_sequential(gpu):
+    from vox8.batch import histories, run_designs
+    from vox8.workloads import c4_designs
+    designs = c4_designs()
+    assert len(designs) == 16 and designs[0].nx == 80 and designs[-1].nx == 220
+    conc = run_designs(designs, steps=2, concurrent=True)
+    seq = run_designs(designs, steps=2, concurrent=False)
+    (cc, fc), (cs, fs) = histories(conc), histories(seq)
+    assert cc.shape == (16, 2)
+    assert rel(cc, cs) < 1e-12 and rel(fc, fs) < 1e-12
+    for a, b in zip(conc, seq):
+        assert rel(a.x.cpu().numpy(), b.x.cpu().numpy()) < 1e-12

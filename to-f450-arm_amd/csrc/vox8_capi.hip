@@ -176,7 +176,7 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   if (w.kmax >= k) return OK;
   dfree(c, w.S); dfree(c, w.AS); dfree(c, w.BS); dfree(c, w.G); dfree(c, w.gram_partials);
   dfree(c, w.res); dfree(c, w.res_partials); dfree(c, w.counters); hfree(c, w.host);
-  dfree(c, w.guard);
+  dfree(c, w.guard); dfree(c, w.conv);
   w.guard_cols = 0;
   const int64_t n = c->g.n_dofs;
   VOX8_TRY(dalloc(c, &w.S, (size_t)3 * k * n));
@@ -194,6 +194,7 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   VOX8_HIP(hipMemset(w.counters, 0, sizeof(unsigned int) * 4));
   VOX8_TRY(halloc(c, &w.host, 32));
   VOX8_TRY(dalloc(c, &w.guard, (size_t)LOBPCG_GUARD_MAX * n));
+  VOX8_TRY(dalloc(c, &w.conv, 16));
   w.kmax = k;
   w.n = n;
   return OK;

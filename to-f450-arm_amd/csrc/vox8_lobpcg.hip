@@ -506,6 +506,46 @@ static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStre
   else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
 }
 
+// Soft locking: a pair whose residual norm fell below tol (sticky, as the
+// host's convergence test) leaves the search space: its W and P columns (and
+// the K P, M P products) are zeroed, so the Rayleigh-Ritz drops them.  Its
+// X column keeps being refined by the other directions.  Without this the
+// noise-level W of converged pairs, normalised by the K scaling, pollutes the
+// basis and can stall the remaining pairs.
+template <int K>
+__global__ void __launch_bounds__(256)
+k_lock(int64_t n, double* __restrict__ S, double* __restrict__ AS, double* __restrict__ BS,
+       const double* __restrict__ res2, int* __restrict__ conv, double tol2) {
+  __shared__ int lk[K];
+  if (threadIdx.x < K) {
+    const int j = threadIdx.x;
+    lk[j] = conv[j] || fabs(res2[j]) < tol2;
+    if (blockIdx.x == 0) conv[j] = lk[j];
+  }
+  __syncthreads();
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (lk[j]) {
+        S[(int64_t)(K + j) * n + r] = 0.0;
+        S[(int64_t)(2 * K + j) * n + r] = 0.0;
+        AS[(int64_t)(2 * K + j) * n + r] = 0.0;
+        BS[(int64_t)(2 * K + j) * n + r] = 0.0;
+      }
+  }
+}
+
+template <int K>
+static void launch_lock(const LobpcgWork& w, double tol2, hipStream_t st) {
+  hipLaunchKernelGGL((k_lock<K>), dim3(grid_blocks(w.n)), dim3(256), 0, st, w.n, w.S, w.AS, w.BS,
+                     (const double*)w.res, w.conv, tol2);
+}
+
+__global__ void k_zero_int(int n, int* __restrict__ a) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a[i] = 0;
+}
+
 template <int K>
 static void launch_residual(const LobpcgWork& w, const double* lam, const double* dinv, hipStream_t st) {
   hipLaunchKernelGGL((k_residual<K>), dim3(grid_blocks(w.n)), dim3(256), 0, st, w.n, w.S,
@@ -523,10 +563,11 @@ static void launch_residual(const LobpcgWork& w, const double* lam, const double
     case 11: F<11>(__VA_ARGS__); break; default: F<12>(__VA_ARGS__); break;                 \
   }
 
-// X' (and P') from the Ritz coefficients, exact AX'/BX' by the stencil, residual.
+// X' (and P') from the Ritz coefficients, exact AX'/BX' by the stencil,
+// residual; tol2 > 0 also soft-locks converged pairs.
 static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
                      const std::function<int(const double*, double*, double*, int, hipStream_t)>& applyKM,
-                     hipStream_t st) {
+                     hipStream_t st, double tol2) {
   const double* C = w.G + 2 * MMAX * MMAX;
   const double* lam = C + MMAX * 12;
   VOX8_K_SWITCH(k, launch_combine, w, nb, C, st);
@@ -534,6 +575,10 @@ static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
   VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st));
   VOX8_K_SWITCH(k, launch_residual, w, lam, dinv, st);
   VOX8_HIP(hipGetLastError());
+  if (tol2 > 0.0) {
+    VOX8_K_SWITCH(k, launch_lock, w, tol2, st);
+    VOX8_HIP(hipGetLastError());
+  }
   return OK;
 }
 
@@ -584,8 +629,11 @@ int lobpcg_fp64(const PcgOps<double>& A,
   // initial Rayleigh-Ritz on X alone
   VOX8_TRY(applyKM(X, w.AS, w.BS, k, st));
   const double* Tdinv = M ? nullptr : A.dinv;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 12, w.conv);
+  VOX8_HIP(hipGetLastError());
+  const double tol2 = tol * tol;
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
-  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st));
+  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st, tol2));
   if (M) VOX8_TRY((*M)(W, W, k, st));
   VOX8_TRY(read_state(w, k, st));
   VOX8_CHECK(w.host[12] == 0.0, "lobpcg: linearly dependent initial block");
@@ -606,7 +654,7 @@ int lobpcg_fp64(const PcgOps<double>& A,
     const int m = have_p ? 3 * k : 2 * k;
     // the Rayleigh-Ritz kernel restarts without P on the device when needed
     VOX8_TRY(gram_and_rr(w, m, k, st, drop_tol));
-    VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st));
+    VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st, tol2));
     if (M) VOX8_TRY((*M)(W, W, k, st));
     have_p = true;
     ++it;
@@ -617,7 +665,7 @@ int lobpcg_fp64(const PcgOps<double>& A,
   // final Rayleigh-Ritz on X alone (scipy lobpcg post-processing); AX, BX
   // are exact products of the current X
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
-  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st));
+  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st, 0.0));
   VOX8_TRY(read_state(w, k, st));
   hipLaunchKernelGGL(k_copy_cols, dim3(grid_blocks(n * kw)), dim3(256), 0, st, n, kw,
                      (const double*)X, Xout);

@@ -67,6 +67,7 @@ struct LobpcgWork {
   double* host = nullptr;     // pinned mirror of lam|res|status
   double* res_partials = nullptr;
   unsigned int* counters = nullptr;
+  int* conv = nullptr;        // device: sticky convergence flags [12] (soft locking)
   double* guard = nullptr;    // last solve's guard vectors [gmax][n] (warm start of the next)
   int guard_cols = 0;         // columns held in `guard` (0 = none)
   int guard_k = 0;            // wanted pairs of the solve that left them

@@ -215,7 +215,8 @@ __device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mo
 //   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r.
 template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE>
 __global__ void __launch_bounds__(NT)
-k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const T* __restrict__ fK,
+k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T> C,
+             const T* __restrict__ fK,
              const T* __restrict__ fM, const uint8_t* __restrict__ fixed,
              const T* __restrict__ x, int64_t ldx, T* __restrict__ yK, T* __restrict__ yM,
              int64_t ldy, DotFuse fuse, Epilogue<T> ep) {
@@ -223,7 +224,16 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
   __shared__ T pl[2][3][PN];          // two node planes, component-major
   __shared__ T xch[NOP][2][3][NT];     // y exchange: s[0][oz][c] per element
   __shared__ double sh[NT / 64];
-  const int col = blockIdx.y;
+  // XCD-aware work order: workgroups are dealt round-robin to the 8 XCDs, so
+  // XCD j = blockIdx.x % 8 takes the contiguous range [j q + min(j, r), ...)
+  // of the nb = 8 q + r work items; an item is (tile, column) with the column
+  // fastest, so the columns of a tile and neighbouring tiles share that XCD's
+  // L2 (element factors, node masks, diagonal, halo nodes)
+  const int nb = gridDim.x, bid = blockIdx.x;
+  const int xq = nb / NXCD, xr = nb % NXCD, xj = bid % NXCD;
+  const int item = xj * xq + min(xj, xr) + bid / NXCD;
+  const int tile = item / ncols, col = item - tile * ncols;
+  const int ntiles = nb / ncols;
   if (FUSE && fuse.scal[col].done) return;
   x += col * ldx;
   if (DO_K && MODE == EPI_PLAIN) yK += col * ldy;
@@ -234,11 +244,6 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
     if (MODE == EPI_RESID) ep.b += col * ep.ld;
     if (MODE == EPI_CHEB || MODE == EPI_CHEB0) ep.xs += col * ep.ld;
   }
-  // XCD-aware tile number: XCD j = bid % 8 gets the contiguous range
-  // [j q + min(j, r), ...) of the nb = 8 q + r tiles
-  const int nb = gridDim.x, bid = blockIdx.x;
-  const int xq = nb / NXCD, xr = nb % NXCD, xj = bid % NXCD;
-  const int tile = xj * xq + min(xj, xr) + bid / NXCD;
   const int tx_t = tile % tiles_x, rest = tile / tiles_x;
   const int ty_t = rest % tiles_y, zc_t = rest / tiles_y;
   const int t = threadIdx.x;
@@ -418,7 +423,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, FastCoef<T> C, const 
   if (FUSE) {
     const double v = block_sum(dot, sh);
     double tot;
-    if (grid_reduce_last<1>(&v, fuse.partials + (int64_t)col * nb, fuse.counters + col, nb, bid,
+    if (grid_reduce_last<1>(&v, fuse.partials + (int64_t)col * ntiles, fuse.counters + col, ntiles, tile,
                             &tot, sh)) {
       if (threadIdx.x == 0) fuse.scal[col].pq = tot;
     }
@@ -461,7 +466,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     return ERR_ARG;
   }
   const int zc = fast_zchunk(g);
-  const dim3 grid(fast_blocks(g), nrhs), blk(NT);
+  const dim3 grid(fast_blocks(g) * nrhs), blk(NT);
   const bool fz = fuse.scal != nullptr;
   if (ep.mode != EPI_PLAIN && (op != OP_K || fz)) {
     set_error("stencil epilogue modes apply to K alone");
@@ -469,7 +474,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
   }
 #define VOX8_LAUNCH(PAT, K, M, F, MD)                                                            \
   hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD>), grid, blk, 0, st, g, zc, tiles_x(g),   \
-                     tiles_y(g), C, fK, fM, fixed, x, ldx, yK, yM, ldy, fuse, ep)
+                     tiles_y(g), nrhs, C, fK, fM, fixed, x, ldx, yK, yM, ldy, fuse, ep)
 #define VOX8_PAT(PAT)                                                                   \
   if (op == OP_K) {                                                                     \
     if (ep.mode == EPI_RESID) VOX8_LAUNCH(PAT, true, false, false, EPI_RESID);          \

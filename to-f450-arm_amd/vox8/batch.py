@@ -1,0 +1,96 @@
+"""Independent designs solved concurrently on one GPU.
+
+The c4 configuration is a volume-fraction sweep of voxel cantilevers of
+different lengths.  The designs share nothing, so each gets its own vox8
+context, HIP stream and host thread: the library's calls release the GIL and
+the designs' kernels overlap on the device, which small designs cannot fill
+alone.  Every kernel is deterministic and the contexts are separate, so each
+design's results are bitwise those of running it alone.  (The reference runs
+one V_f per PBS job, sims/run_vf_gpu.py.)
+"""
+from __future__ import annotations
+
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import torch
+
+from .fem import FEM3D, Material
+from .topopt import SIMPOptimizer
+from .workloads import E_PA, NU, RHO_MAT, Workload, load_cases, seeded_density
+
+__all__ = ["simp_for_workload", "make_designs", "step_designs", "run_designs", "histories"]
+
+
+def simp_for_workload(wl: Workload, precision: str = "mixed", min_freq_Hz: float = 1e4,
+                      modal_maxiter: int = 2000) -> SIMPOptimizer:
+    """A started SIMP loop for a synthetic workload: seeded density, its load
+    cases, wl.n_modes tracked modes, frequency target min_freq_Hz (above f1
+    keeps the eigenvalue adjoint active), the workload's solver tolerances."""
+    geom = wl.geometry()
+    fem = FEM3D(geom, Material(E=E_PA, nu=NU, rho=RHO_MAT), precision=precision)
+    F = load_cases(geom, wl.loads)
+    opt = SIMPOptimizer(geom, fem, load_cases=[{}] * len(F), vol_frac=wl.base,
+                        r_min_mm=wl.r_min_vox * geom.dx * 1e3, max_iter=10 ** 6,
+                        min_freq_Hz=min_freq_Hz, tip_mass_kg=0.0, n_modes_track=wl.n_modes,
+                        F_list=list(F), solve_rtol=wl.pcg_rtol, modal_tol=wl.modal_tol,
+                        modal_maxiter=modal_maxiter, solve_maxiter=40000)
+    return opt.start(rho_init=seeded_density(geom, wl.base, wl.amp, wl.seed))
+
+
+class Design:
+    """One design of a batch: its SIMP loop and the stream it runs on."""
+
+    def __init__(self, wl: Workload, precision: str, concurrent: bool):
+        self.wl = wl
+        self.stream = torch.cuda.Stream() if concurrent else torch.cuda.current_stream()
+        with torch.cuda.stream(self.stream):
+            self.opt = simp_for_workload(wl, precision)
+
+    def step(self, n: int):
+        with torch.cuda.stream(self.stream):
+            for _ in range(n):
+                self.opt.step()
+            self.stream.synchronize()
+
+
+def _map(fn, items, concurrent, max_workers):
+    if not concurrent:
+        return [fn(it) for it in items]
+    device = torch.cuda.current_device()
+
+    def run(it):
+        torch.cuda.set_device(device)
+        return fn(it)
+
+    with ThreadPoolExecutor(max_workers=min(max_workers, len(items))) as ex:
+        return list(ex.map(run, items))
+
+
+def make_designs(workloads, precision: str = "mixed", concurrent: bool = True,
+                 max_workers: int = 16):
+    """Set up one SIMP loop per workload (concurrently when asked)."""
+    return _map(lambda wl: Design(wl, precision, concurrent), list(workloads), concurrent,
+                max_workers)
+
+
+def step_designs(designs, steps: int, concurrent: bool = True, max_workers: int = 16):
+    """Advance every design by `steps` SIMP iterations; concurrent=False runs
+    them one after another (the reference behaviour, for comparison)."""
+    _map(lambda d: d.step(steps), list(designs), concurrent, max_workers)
+    return designs
+
+
+def run_designs(workloads, steps: int, precision: str = "mixed", concurrent: bool = True,
+                max_workers: int = 16):
+    """Set up and run `steps` SIMP iterations of every workload; returns the
+    optimizers in input order."""
+    ds = make_designs(workloads, precision, concurrent, max_workers)
+    step_designs(ds, steps, concurrent, max_workers)
+    return [d.opt for d in ds]
+
+
+def histories(opts):
+    """Compliance and first-frequency histories, (n_designs, steps) arrays."""
+    return (np.array([o.history["compliance"] for o in opts]),
+            np.array([o.history["omega_1_Hz"] for o in opts]))
