@@ -120,7 +120,7 @@ int ensure_dinv(vox8_ctx* c, int guard) {
 
 template <typename T>
 int apply_op(vox8_ctx* c, int op, const T* x, int64_t ldx, T* yK, T* yM, int64_t ldy, int nrhs,
-             DotFuse fuse) {
+             DotFuse fuse, const int* skip = nullptr) {
   const T* fK = nullptr;
   const T* fM = nullptr;
   const T* Ke = nullptr;
@@ -131,12 +131,14 @@ int apply_op(vox8_ctx* c, int op, const T* x, int64_t ldx, T* yK, T* yM, int64_t
     fK = (const T*)c->fK32; fM = (const T*)c->fM32; Ke = (const T*)c->dKe32; Me = (const T*)c->dMe32;
   }
   if (c->pattern != PAT_DENSE && c->stencil_kind == 0) {
+    Epilogue<T> ep;
+    ep.skip = skip;
     if constexpr (sizeof(T) == 8)
       VOX8_TRY(launch_apply_fast<double>(c->g, c->pattern, op, c->c64, fK, fM, c->d_fixed, x, ldx,
-                                         yK, yM, ldy, nrhs, c->st, fuse));
+                                         yK, yM, ldy, nrhs, c->st, fuse, ep));
     else
       VOX8_TRY(launch_apply_fast<float>(c->g, c->pattern, op, c->c32, fK, fM, c->d_fixed, x, ldx,
-                                        yK, yM, ldy, nrhs, c->st, fuse));
+                                        yK, yM, ldy, nrhs, c->st, fuse, ep));
   } else {
     VOX8_TRY(launch_apply_dense<T>(c->g, op, fK, fM, Ke, Me, c->d_fixed, x, ldx, yK, yM, ldy, nrhs,
                                    c->st, fuse));
@@ -150,7 +152,7 @@ template <typename T>
 int ensure_pcg_work(vox8_ctx* c, PcgWork<T>& w, int ncols) {
   if (w.ncols >= ncols) return OK;
   dfree(c, w.r); dfree(c, w.p); dfree(c, w.q); dfree(c, w.x); dfree(c, w.rhs64);
-  dfree(c, w.scal); dfree(c, w.partials); dfree(c, w.counters);
+  dfree(c, w.scal); dfree(c, w.partials); dfree(c, w.counters); dfree(c, w.skip);
   hfree(c, w.host_scal); hfree(c, w.host_scale);
   const int64_t n = c->g.n_dofs;
   VOX8_TRY(dalloc(c, &w.r, (size_t)ncols * n));
@@ -159,6 +161,7 @@ int ensure_pcg_work(vox8_ctx* c, PcgWork<T>& w, int ncols) {
   VOX8_TRY(dalloc(c, &w.x, (size_t)ncols * n));
   VOX8_TRY(dalloc(c, &w.rhs64, (size_t)ncols * n + ncols));
   VOX8_TRY(dalloc(c, &w.scal, ncols));
+  VOX8_TRY(dalloc(c, &w.skip, ncols));
   const int64_t apply_blocks = (c->g.n_nodes + 255) / 256;
   const int64_t per_col = apply_blocks > 3 * 1024 ? apply_blocks : 3 * 1024;
   VOX8_TRY(dalloc(c, &w.partials, (size_t)(ncols * per_col)));
@@ -517,8 +520,8 @@ int vox8_pcg(vox8_ctx* c, const double* d_F, double* d_U, int nrhs, int64_t ld, 
                   DotFuse f) { return apply_op<double>(c, OP_K, x, ldx, y, nullptr, ldy, nr, f); };
   if (prec == VOX8_PREC_MIXED) {
     VOX8_TRY(ensure_mg(c, nrhs));
-    Precond<double> M = [c](const double* r, double* z, int nr, hipStream_t st) {
-      return mg_apply(c->mg, r, z, nr, st);
+    Precond<double> M = [c](const double* r, double* z, int nr, hipStream_t st, const int* skip) {
+      return mg_apply(c->mg, r, z, nr, st, skip);
     };
     VOX8_TRY(pcg_fp64(A64, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, c->w64, c->st,
                       res.data(), &M));
@@ -559,14 +562,14 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
   A.g = &c->g;
   A.fixed = c->d_fixed;
   A.dinv = c->dinvK;
-  auto km = [c](const double* x, double* yK, double* yM, int nr, hipStream_t) {
-    return apply_op<double>(c, OP_KM, x, c->g.n_dofs, yK, yM, c->g.n_dofs, nr, DotFuse());
+  auto km = [c](const double* x, double* yK, double* yM, int nr, hipStream_t, const int* skip) {
+    return apply_op<double>(c, OP_KM, x, c->g.n_dofs, yK, yM, c->g.n_dofs, nr, DotFuse(), skip);
   };
   LobpcgResult r{};
   if (prec == VOX8_PREC_MIXED) {
     VOX8_TRY(ensure_mg(c, k + guard));
-    Precond<double> M = [c](const double* rr, double* z, int nr, hipStream_t st) {
-      return mg_apply(c->mg, rr, z, nr, st);
+    Precond<double> M = [c](const double* rr, double* z, int nr, hipStream_t st, const int* skip) {
+      return mg_apply(c->mg, rr, z, nr, st, skip);
     };
     VOX8_TRY(lobpcg_fp64(A, km, k, guard, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed, &M));
   } else {

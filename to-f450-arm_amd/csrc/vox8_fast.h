@@ -56,6 +56,7 @@ struct Epilogue {
   const T* dinv = nullptr;   // Jacobi inverse diagonal (one vector, no ld)
   T c0 = 0, c1 = 0, c2 = 0;
   int64_t ld = 0;
+  const int* skip = nullptr;  // per column: non-zero = leave this column untouched
 };
 
 int fast_zchunk(const Grid& g);

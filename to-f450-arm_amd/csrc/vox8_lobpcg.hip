@@ -507,8 +507,9 @@ static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStre
 }
 
 // Soft locking: a pair whose residual norm fell below tol (sticky, as the
-// host's convergence test) leaves the search space: its W and P columns (and
-// the K P, M P products) are zeroed, so the Rayleigh-Ritz drops them.  Its
+// host's convergence test) leaves the search space: its W and P columns and
+// their K, M products are zeroed, so the Rayleigh-Ritz drops them, and the
+// preconditioner and the K, M products of W skip the column.  Its
 // X column keeps being refined by the other directions.  Without this the
 // noise-level W of converged pairs, normalised by the K scaling, pollutes the
 // basis and can stall the remaining pairs.
@@ -530,6 +531,8 @@ k_lock(int64_t n, double* __restrict__ S, double* __restrict__ AS, double* __res
       if (lk[j]) {
         S[(int64_t)(K + j) * n + r] = 0.0;
         S[(int64_t)(2 * K + j) * n + r] = 0.0;
+        AS[(int64_t)(K + j) * n + r] = 0.0;
+        BS[(int64_t)(K + j) * n + r] = 0.0;
         AS[(int64_t)(2 * K + j) * n + r] = 0.0;
         BS[(int64_t)(2 * K + j) * n + r] = 0.0;
       }
@@ -566,13 +569,12 @@ static void launch_residual(const LobpcgWork& w, const double* lam, const double
 // X' (and P') from the Ritz coefficients, exact AX'/BX' by the stencil,
 // residual; tol2 > 0 also soft-locks converged pairs.
 static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
-                     const std::function<int(const double*, double*, double*, int, hipStream_t)>& applyKM,
-                     hipStream_t st, double tol2) {
+                     const ApplyKM& applyKM, hipStream_t st, double tol2) {
   const double* C = w.G + 2 * MMAX * MMAX;
   const double* lam = C + MMAX * 12;
   VOX8_K_SWITCH(k, launch_combine, w, nb, C, st);
   VOX8_HIP(hipGetLastError());
-  VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st));
+  VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st, nullptr));
   VOX8_K_SWITCH(k, launch_residual, w, lam, dinv, st);
   VOX8_HIP(hipGetLastError());
   if (tol2 > 0.0) {
@@ -592,8 +594,7 @@ static int read_state(LobpcgWork& w, int k, hipStream_t st) {
   return OK;
 }
 
-int lobpcg_fp64(const PcgOps<double>& A,
-                const std::function<int(const double*, double*, double*, int, hipStream_t)>& applyKM,
+int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
                 int kw, int guard, double tol, int maxiter, const double* X0, double* Xout,
                 double* lam_out, LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
                 const Precond<double>* M) {
@@ -627,14 +628,14 @@ int lobpcg_fp64(const PcgOps<double>& A,
     VOX8_HIP(hipGetLastError());
   }
   // initial Rayleigh-Ritz on X alone
-  VOX8_TRY(applyKM(X, w.AS, w.BS, k, st));
+  VOX8_TRY(applyKM(X, w.AS, w.BS, k, st, nullptr));
   const double* Tdinv = M ? nullptr : A.dinv;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 12, w.conv);
   VOX8_HIP(hipGetLastError());
   const double tol2 = tol * tol;
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
   VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st, tol2));
-  if (M) VOX8_TRY((*M)(W, W, k, st));
+  if (M) VOX8_TRY((*M)(W, W, k, st, w.conv));
   VOX8_TRY(read_state(w, k, st));
   VOX8_CHECK(w.host[12] == 0.0, "lobpcg: linearly dependent initial block");
   std::vector<int> conv(kw, 0);
@@ -650,12 +651,12 @@ int lobpcg_fp64(const PcgOps<double>& A,
   int it = 0;
   bool have_p = false;
   while (!all && it < maxiter) {
-    VOX8_TRY(applyKM(W, AW, BW, k, st));
+    VOX8_TRY(applyKM(W, AW, BW, k, st, w.conv));   // locked pairs: W, KW, MW stay zero
     const int m = have_p ? 3 * k : 2 * k;
     // the Rayleigh-Ritz kernel restarts without P on the device when needed
     VOX8_TRY(gram_and_rr(w, m, k, st, drop_tol));
     VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st, tol2));
-    if (M) VOX8_TRY((*M)(W, W, k, st));
+    if (M) VOX8_TRY((*M)(W, W, k, st, w.conv));
     have_p = true;
     ++it;
     VOX8_TRY(read_state(w, k, st));

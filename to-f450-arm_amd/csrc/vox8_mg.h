@@ -62,7 +62,9 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
 // Ensure work vectors for ncols columns.
 int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al);
 // z = cycle(r) (V or W, see MultiGrid::gamma) for ncols FP64 column-major
-// vectors (ld = fine n_dofs).
-int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st);
+// vectors (ld = fine n_dofs).  skip (device, per column, may be null):
+// non-zero columns are left untouched (z keeps its contents).
+int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,
+             const int* skip = nullptr);
 
 }  // namespace vox8

@@ -73,10 +73,12 @@ __global__ void k_mg_jacobi(Grid g, const double* __restrict__ f, Diag24 dg,
 
 // coarse b = R r: weights 1 at the coincident fine node, 1/2 at the neighbours, per axis
 __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
-                              const uint8_t* __restrict__ fixc, float* __restrict__ bc) {
+                              const uint8_t* __restrict__ fixc, float* __restrict__ bc,
+                              const int* __restrict__ skip) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= gc.n_nodes) return;
   const int col = blockIdx.y;
+  if (skip && skip[col]) return;
   rf += (int64_t)col * gf.n_dofs;
   bc += (int64_t)col * gc.n_dofs;
   const int I = (int)(n % gc.NX), J = (int)((n / gc.NX) % gc.NY), K = (int)(n / ((int64_t)gc.NX * gc.NY));
@@ -104,10 +106,12 @@ __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
 
 // fine x += P coarse x (trilinear), fixed fine DOFs untouched
 __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
-                                 const uint8_t* __restrict__ fixf, float* __restrict__ xf) {
+                                 const uint8_t* __restrict__ fixf, float* __restrict__ xf,
+                                 const int* __restrict__ skip) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= gf.n_nodes) return;
   const int col = blockIdx.y;
+  if (skip && skip[col]) return;
   xc += (int64_t)col * gc.n_dofs;
   xf += (int64_t)col * gf.n_dofs;
   const int i = (int)(n % gf.NX), j = (int)((n / gf.NX) % gf.NY), k = (int)(n / ((int64_t)gf.NX * gf.NY));
@@ -204,10 +208,11 @@ __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __res
 // the wavefront.  16 rows per block, fixed summation order.
 __global__ void __launch_bounds__(256)
 k_mg_coarse_apply(int n, const float* __restrict__ Ainv, const float* __restrict__ b,
-                  float* __restrict__ x) {
+                  float* __restrict__ x, const int* __restrict__ skip) {
   const int lane = threadIdx.x & 63;
   const int row0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
   const int col = blockIdx.y;
+  if (skip && skip[col]) return;
   const float* bc = b + (int64_t)col * n;
   float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int k = lane; k < n; k += 64) {
@@ -225,14 +230,18 @@ k_mg_coarse_apply(int n, const float* __restrict__ Ainv, const float* __restrict
   }
 }
 
-__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+__global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b,
+                             const int* __restrict__ skip) {
+  if (skip && skip[blockIdx.y]) return;
   const int64_t off = (int64_t)blockIdx.y * n;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     b[off + i] = (float)a[off + i];
 }
 
-__global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+__global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b,
+                             const int* __restrict__ skip) {
+  if (skip && skip[blockIdx.y]) return;
   const int64_t off = (int64_t)blockIdx.y * n;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -478,7 +487,8 @@ struct ChebCoef { double theta, delta, sigma; };
 // at x = 0, r = b and the first step (EPI_CHEB0) reads b and forms the first
 // direction itself; otherwise the direction left in L.d by a residual pass is
 // continued from the current x, r.
-static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st, bool from_zero) {
+static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st, bool from_zero,
+                  const int* skip) {
   const double lmax = L.lam_max, lmin = lmax / alpha;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
   double rho = 1.0 / sigma;
@@ -496,6 +506,7 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
     ep.c1 = (float)(rho1 * rho);
     ep.c2 = (float)(2.0 * rho1 / delta);
     ep.ld = L.g.n_dofs;
+    ep.skip = skip;
     VOX8_TRY(level_apply(L, in, nullptr, ncols, st, ep));
     in = out;
     out = (out == L.d) ? L.d2 : L.d;
@@ -511,28 +522,28 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
 // 8 nu smoothing steps from zero when it is too large for one.  With the same
 // polynomial before and after and symmetric transfers the operator is
 // symmetric, as PCG requires.
-static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st) {
+static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* skip) {
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
     hipLaunchKernelGGL(k_mg_coarse_apply, dim3((mg.n_coarse + 15) / 16, ncols), dim3(256), 0, st,
-                       mg.n_coarse, (const float*)mg.Ainv, (const float*)L.b, L.x);
+                       mg.n_coarse, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
     VOX8_HIP(hipGetLastError());
     return OK;
   }
   const double theta = 0.5 * (L.lam_max + L.lam_max / mg.alpha);
-  if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true);   // no coarse grid below
-  VOX8_TRY(smooth(L, mg.nu, mg.alpha, ncols, st, true));
+  if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true, skip);   // no coarse grid below
+  VOX8_TRY(smooth(L, mg.nu, mg.alpha, ncols, st, true, skip));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
   const int g = (exact_below || L.g.n_dofs < mg.w_min_dofs) ? 1 : mg.gamma;
   for (int j = 0; j < g; ++j) {
     hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
-                       L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b);
+                       L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b, skip);
     VOX8_HIP(hipGetLastError());
-    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st));
+    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip));
     hipLaunchKernelGGL(k_mg_prolong_add, dim3((unsigned)((L.g.n_nodes + 255) / 256), ncols), dim3(256), 0,
-                       st, Cn.g, L.g, (const float*)Cn.x, (const uint8_t*)L.fixed, L.x);
+                       st, Cn.g, L.g, (const float*)Cn.x, (const uint8_t*)L.fixed, L.x, skip);
     VOX8_HIP(hipGetLastError());
     Epilogue<float> ep;
     ep.mode = EPI_RESID;
@@ -542,19 +553,21 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st) {
     ep.dinv = L.dinv;
     ep.c2 = (float)(1.0 / theta);
     ep.ld = L.g.n_dofs;
+    ep.skip = skip;
     VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
   }
-  return smooth(L, mg.nu, mg.alpha, ncols, st, false);
+  return smooth(L, mg.nu, mg.alpha, ncols, st, false, skip);
 }
 
-int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st) {
+int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st, const int* skip) {
   VOX8_CHECK(mg.updated && ncols <= mg.ncols, "multigrid not set up for this density / width");
   MGLevel& F = mg.lv[0];
-  hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b);
+  hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b,
+                     skip);
   VOX8_HIP(hipGetLastError());
-  VOX8_TRY(mg_cycle(mg, 0, ncols, st));
+  VOX8_TRY(mg_cycle(mg, 0, ncols, st, skip));
   hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
-                     (const float*)F.x, z);
+                     (const float*)F.x, z, skip);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

@@ -24,6 +24,7 @@ struct PcgWork {
   T* q = nullptr;
   T* x = nullptr;                 // only used as the correction of the mixed solve
   double* rhs64 = nullptr;        // FP64 right-hand side of the inner solve (+nrhs scales)
+  int* skip = nullptr;            // device [ncols]: column finished (multigrid skips it)
   PcgScalars* scal = nullptr;     // device [ncols]
   PcgScalars* host_scal = nullptr;  // pinned [ncols]
   double* host_scale = nullptr;     // pinned [ncols]
@@ -33,9 +34,15 @@ struct PcgWork {
   int64_t n = 0;
 };
 
-// Preconditioner z = M r for nrhs column-major vectors (ld = n_dofs); r and z may alias.
+// Preconditioner z = M r for nrhs column-major vectors (ld = n_dofs); r and z
+// may alias.  skip (device, per column, may be null): non-zero columns are
+// left untouched.
 template <typename T>
-using Precond = std::function<int(const T* r, T* z, int nrhs, hipStream_t st)>;
+using Precond = std::function<int(const T* r, T* z, int nrhs, hipStream_t st, const int* skip)>;
+
+// K and M products of nrhs columns; skip as above (those outputs untouched).
+using ApplyKM = std::function<int(const double* x, double* yK, double* yM, int nrhs, hipStream_t st,
+                                  const int* skip)>;
 
 struct PcgResult {
   int iters;      // CG iterations (summed over inner solves for the mixed path)
@@ -80,8 +87,7 @@ constexpr int LOBPCG_GUARD_MAX = 2;
 
 // k wanted pairs, iterated with `guard` extra columns (k + guard <= 12); only
 // the k lowest are tested for convergence and returned.
-int lobpcg_fp64(const PcgOps<double>& A, const std::function<int(const double*, double*, double*,
-                                                                 int, hipStream_t)>& applyKM,
+int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
                 int k, int guard, double tol, int maxiter, const double* X0, double* Xout,
                 double* lam_out, LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
                 const Precond<double>* M = nullptr);

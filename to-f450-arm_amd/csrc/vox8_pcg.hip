@@ -157,10 +157,12 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
 }
 
 // Preconditioned variant (explicit z = M r): stop test, then rho = r.z.
-__global__ void k_pcg_check(PcgScalars* __restrict__ scal, int maxiter) {
+// skip[col] tells the preconditioner which columns need no more work.
+__global__ void k_pcg_check(PcgScalars* __restrict__ scal, int maxiter, int* __restrict__ skip) {
   const int col = blockIdx.y;
   PcgScalars& s = scal[col];
   if (!s.done && s.iters < maxiter && s.rr < s.atol2) s.done = 1;
+  skip[col] = s.done || s.iters >= maxiter;
 }
 
 template <typename T>
@@ -272,8 +274,8 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
         hipLaunchKernelGGL((k_pcg_step<T, true>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
                            w.scal, maxiter, w.partials, w.counters);
       } else {
-        hipLaunchKernelGGL(k_pcg_check, dim3(1, nrhs), dim3(1), 0, st, w.scal, maxiter);
-        VOX8_TRY((*M)(w.r, w.x, nrhs, st));
+        hipLaunchKernelGGL(k_pcg_check, dim3(1, nrhs), dim3(1), 0, st, w.scal, maxiter, w.skip);
+        VOX8_TRY((*M)(w.r, w.x, nrhs, st, w.skip));
         hipLaunchKernelGGL((k_pcg_rz<T>), grid, blk, 0, st, n, (const T*)w.r, (const T*)w.x, n,
                            w.scal, maxiter, w.partials, w.counters);
         hipLaunchKernelGGL((k_pcg_dir_z<T>), grid, blk, 0, st, n, (const T*)w.x, w.p, n,

@@ -235,6 +235,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const int tile = item / ncols, col = item - tile * ncols;
   const int ntiles = nb / ncols;
   if (FUSE && fuse.scal[col].done) return;
+  if (ep.skip && ep.skip[col]) return;
   x += col * ldx;
   if (DO_K && MODE == EPI_PLAIN) yK += col * ldy;
   if (DO_M) yM += col * ldy;
