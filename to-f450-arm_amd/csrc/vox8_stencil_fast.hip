@@ -109,6 +109,39 @@ __device__ __forceinline__ void inv8(const T m[8], T v[8]) {
   v[4] = s2 - d2; v[5] = s2 + d2; v[6] = s3 - d3; v[7] = s3 + d3;
 }
 
+// A node's three components as one access.
+template <typename T>
+struct Vec3 {
+  T a, b, c;
+  __device__ __forceinline__ T operator[](int i) const { return i == 0 ? a : (i == 1 ? b : c); }
+};
+template <typename T>
+__device__ __forceinline__ Vec3<T> ld3(const T* __restrict__ p, uint32_t node) {
+  return *reinterpret_cast<const Vec3<T>*>(p + 3u * node);
+}
+// Lanes without an owned node write to a sink instead of skipping the store,
+// so the stores are not in a branch (see the loop).
+__device__ double store_sink[4];
+template <typename T>
+__device__ __forceinline__ void st3(T* dst, Vec3<T>* sink, const T v[3]) {
+  Vec3<T>* p = dst ? reinterpret_cast<Vec3<T>*>(dst) : sink;
+  *p = Vec3<T>{v[0], v[1], v[2]};
+}
+
+// Value of lane + 1 of the wavefront (DPP wave_shl:1, one VALU move per
+// dword; lane 63 receives no value and its result is never used).
+__device__ __forceinline__ float lane_next(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130,
+                                                               0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_next(double v) {
+  const int2 w = __builtin_bit_cast(int2, v);
+  int2 r;
+  r.x = __builtin_amdgcn_update_dpp(0, w.x, 0x130, 0xf, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp(0, w.y, 0x130, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, r);
+}
+
 // corner k = ox + 2 oy + 4 oz of the element.  After the x exchange a thread
 // holds, per face oz and row oy, the sum of its element's ox = 1 corner and the
 // right neighbour's ox = 0 corner: s[oy][oz] belongs to node (ex + 1, ey + oy).
@@ -121,7 +154,7 @@ __device__ __forceinline__ void x_exchange(const T f[3][8], T s[2][2][3]) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int k0 = 2 * oy + 4 * oz;
-        s[oy][oz][c] = f[c][k0 + 1] + __shfl_down(f[c][k0], 1, 64);
+        s[oy][oz][c] = f[c][k0 + 1] + lane_next(f[c][k0]);
       }
 }
 
@@ -214,15 +247,19 @@ __device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mo
 // EPI_CHEB0 the same step from xs = 0, r = b with x holding b: d = c0 D^-1 b
 //   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r.
 template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !DO_M ? 6 : 1)))
 k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T> C,
              const T* __restrict__ fK,
              const T* __restrict__ fM, const uint8_t* __restrict__ fixed,
              const T* __restrict__ x, int64_t ldx, T* __restrict__ yK, T* __restrict__ yM,
              int64_t ldy, DotFuse fuse, Epilogue<T> ep) {
   constexpr int NOP = (DO_K ? 1 : 0) + (DO_M ? 1 : 0);
-  __shared__ T pl[2][3][PN];          // two node planes, component-major
-  __shared__ T xch[NOP][2][3][NT];     // y exchange: s[0][oz][c] per element
+  // FP32: three plane slots and two y-exchange buffers, one barrier per layer.
+  // FP64 (twice the LDS per value): two slots, one buffer, two barriers.
+  constexpr bool ONEBAR = sizeof(T) == 4;
+  constexpr int NSLOT = ONEBAR ? 3 : 2, NXB = ONEBAR ? 2 : 1;
+  __shared__ T pl[NSLOT][3][PN];       // node planes, component-major
+  __shared__ T xch[NXB][NOP][2][3][NT];  // y exchange: s[0][oz][c] per element
   __shared__ double sh[NT / 64];
   // XCD-aware work order: workgroups are dealt round-robin to the 8 XCDs, so
   // XCD j = blockIdx.x % 8 takes the contiguous range [j q + min(j, r), ...)
@@ -273,28 +310,31 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     qin[i] = q < PN && nxg >= 0 && nxg < g.NX && nyg >= 0 && nyg < g.NY;
     qoff[i] = qin[i] ? (uint32_t)nxg + (uint32_t)g.NX * (uint32_t)nyg : 0u;
   }
+  // Memory operations of the loop.  Every global load is issued one layer
+  // before its use: at the top of layer L the node plane L + 2 and the
+  // element/epilogue operands of layer L + 1.  No global load or store sits in
+  // a branch: out-of-range loads read node/element 0 and their value is
+  // discarded, and lanes that own no node store to a sink.  (The memory
+  // counter covers loads and stores in issue order; with path-dependent counts
+  // the compiler's waits for a load would also wait for just-issued stores.)
+  // Node triples move as one 3-dword (FP32) / 3-qword (FP64) access per lane.
   T pre[NLD][3];
+  T prd[NLD][3];                                               // EPI_CHEB0: D^-1 of the plane
+  uint32_t prf[NLD];
+  bool pin[NLD];
   auto fetch = [&](int iz) {
     const bool zin = iz >= 0 && iz < g.NZ;
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
-      T v0 = T(0), v1 = T(0), v2 = T(0);
-      if (zin && qin[i]) {
-        const uint32_t n = qoff[i] + plane * (uint32_t)iz;
-        const uint32_t f = fixed[n];
-        v0 = x[3u * n];
-        v1 = x[3u * n + 1u];
-        v2 = x[3u * n + 2u];
-        if (MODE == EPI_CHEB0) {
-          v0 *= ep.c0 * ep.dinv[3u * n];
-          v1 *= ep.c0 * ep.dinv[3u * n + 1u];
-          v2 *= ep.c0 * ep.dinv[3u * n + 2u];
-        }
-        v0 = (f & 1u) ? T(0) : v0;
-        v1 = (f & 2u) ? T(0) : v1;
-        v2 = (f & 4u) ? T(0) : v2;
+      pin[i] = zin && qin[i];
+      const uint32_t n = pin[i] ? qoff[i] + plane * (uint32_t)iz : 0u;
+      prf[i] = fixed[n];
+      const Vec3<T> v = ld3(x, n);
+      pre[i][0] = v.a; pre[i][1] = v.b; pre[i][2] = v.c;
+      if (MODE == EPI_CHEB0) {
+        const Vec3<T> d = ld3(ep.dinv, n);
+        prd[i][0] = d.a; prd[i][1] = d.b; prd[i][2] = d.c;
       }
-      pre[i][0] = v0; pre[i][1] = v1; pre[i][2] = v2;
     }
   };
   auto store = [&](int slot) {
@@ -303,44 +343,72 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
       const int q = t + i * NT;
       if (q < PN)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pl[slot][c][q] = pre[i][c];
+        for (int c = 0; c < 3; ++c) {
+          T v = pre[i][c];
+          if (MODE == EPI_CHEB0) v *= ep.c0 * prd[i][c];
+          pl[slot][c][q] = (pin[i] && !((prf[i] >> c) & 1u)) ? v : T(0);
+        }
     }
   };
   const uint32_t exy = exy_in ? (uint32_t)ex + (uint32_t)g.nx * (uint32_t)ey : 0u;
   const uint32_t eplane = (uint32_t)g.nx * (uint32_t)g.ny;
   const uint32_t nxy = own_in ? (uint32_t)gx + (uint32_t)g.NX * (uint32_t)gy : 0u;
 
+  // operands of one layer: element factors of element layer L and, for the
+  // owned node of node plane L, its mask and the epilogue inputs
+  struct LayerOps {
+    T fk, fm;
+    uint32_t fnode;
+    Vec3<T> a;    // RESID: b; CHEB: r; CHEB0: b (= x)
+    Vec3<T> di;   // D^-1
+    Vec3<T> xs;   // CHEB: iterate
+  };
+  auto load_ops = [&](int L, LayerOps& o) {
+    const bool ein = exy_in && L >= 0 && L < g.nz;
+    const uint32_t e = ein ? exy + eplane * (uint32_t)L : 0u;
+    if (DO_K) o.fk = fK[e];
+    if (DO_M) o.fm = fM[e];
+    const bool ok = own_in && L >= zs && L < ze;
+    const uint32_t nn = ok ? nxy + plane * (uint32_t)L : 0u;
+    o.fnode = fixed[nn];
+    if (MODE == EPI_RESID) o.a = ld3(ep.b, nn);
+    if (MODE == EPI_CHEB) o.a = ld3(ep.r, nn);
+    if (MODE == EPI_CHEB0) o.a = ld3(x, nn);
+    if (MODE != EPI_PLAIN) o.di = ld3(ep.dinv, nn);
+    if (MODE == EPI_CHEB) o.xs = ld3(ep.xs, nn);
+  };
+  Vec3<T>* const sink = reinterpret_cast<Vec3<T>*>(store_sink);
+
   T carK[3] = {0, 0, 0}, carM[3] = {0, 0, 0};
   double dot = 0.0;
+  LayerOps cur, nxt;
   fetch(zs - 1);
   store(0);
   fetch(zs);
   store(1);
+  if (ONEBAR) fetch(zs + 1);
+  load_ops(zs - 1, cur);
+  // All loads of the prologue complete here, so none is pending at the loop
+  // header.
+  __builtin_amdgcn_s_waitcnt(0x0F70);                          // vmcnt(0)
   __syncthreads();
+  const int tn = owner ? t + TX : t;                           // element (tx, ty + 1)
+  // slots of planes L (sa), L + 1 (sb) and, one barrier: L + 2 (sc)
+  int sa = 0, sb = 1, sc = 2, xb = 0;
   for (int L = zs - 1; L < ze; ++L) {
-    const int s0 = (L - zs + 1) & 1;                            // slot of plane L
-    if (L + 2 <= ze) fetch(L + 2);
-    T fk = T(0), fm = T(0);
-    if (exy_in && L >= 0 && L < g.nz) {
-      const uint32_t e = exy + eplane * (uint32_t)L;
-      if (DO_K) fk = fK[e];
-      if (DO_M) fm = fM[e];
-    }
-    // owned-node operands of the epilogue (plane L), loaded early
+    // One barrier: plane L + 2 (loaded during layer L - 1) goes to the slot of
+    // plane L - 1, which every thread finished reading before the previous
+    // barrier; it is read after this layer's barrier.
+    if (ONEBAR) store(sc);
+    // unconditional (past the chunk the addresses fall back to 0 and the
+    // values are unused)
+    load_ops(L + 1, nxt);
+    fetch(ONEBAR ? L + 3 : L + 2);
+    const bool ein = exy_in && L >= 0 && L < g.nz;
+    const T fk = (DO_K && ein) ? cur.fk : T(0);
+    const T fm = (DO_M && ein) ? cur.fm : T(0);
     const bool out_ok = own_in && L >= zs;
     const uint32_t nn = nxy + plane * (uint32_t)(L < 0 ? 0 : L);
-    T eb[3] = {0, 0, 0}, er[3] = {0, 0, 0}, ed[3] = {0, 0, 0};
-    uint32_t fnode = 0;
-    if (out_ok) {
-      fnode = fixed[nn];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (MODE == EPI_RESID) eb[c] = ep.b[3u * nn + c];
-        if (MODE == EPI_CHEB) er[c] = ep.r[3u * nn + c];
-        if (MODE == EPI_CHEB0) er[c] = x[3u * nn + c];          // r = b
-        if (MODE != EPI_PLAIN) ed[c] = ep.dinv[3u * nn + c];
-      }
-    }
     T mono[3][8];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -348,13 +416,13 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-        v[k] = pl[s0 ^ oz][c][(ty + oy) * PX + tx + ox];
+        v[k] = pl[oz ? sb : sa][c][(ty + oy) * PX + tx + ox];
       }
       fwd8(v, mono[c]);
     }
     T xin[3];                                                   // owned node's input value
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xin[c] = pl[s0][c][(ty + 1) * PX + tx + 1];
+    for (int c = 0; c < 3; ++c) xin[c] = pl[sa][c][(ty + 1) * PX + tx + 1];
     T sK[2][2][3], sM[2][2][3];
     if (DO_K) {
       T f[3][8];
@@ -363,7 +431,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
 #pragma unroll
       for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xch[0][oz][c][t] = sK[0][oz][c];
+        for (int c = 0; c < 3; ++c) xch[xb][0][oz][c][t] = sK[0][oz][c];
     }
     if (DO_M) {
       T f[3][8];
@@ -372,54 +440,70 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
 #pragma unroll
       for (int oz = 0; oz < 2; ++oz)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xch[NOP - 1][oz][c][t] = sM[0][oz][c];
+        for (int c = 0; c < 3; ++c) xch[xb][NOP - 1][oz][c][t] = sM[0][oz][c];
     }
     __syncthreads();
-    if (owner) {
-      const int tn = t + TX;                                    // element (tx, ty + 1)
-      T bK[3], bM[3];
+    if (!ONEBAR) store(sa);                                     // plane L + 2 replaces plane L
+    T bK[3], bM[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (DO_K) {
-          bK[c] = carK[c] + sK[1][0][c] + xch[0][0][c][tn];
-          carK[c] = sK[1][1][c] + xch[0][1][c][tn];
-        }
-        if (DO_M) {
-          bM[c] = carM[c] + sM[1][0][c] + xch[NOP - 1][0][c][tn];
-          carM[c] = sM[1][1][c] + xch[NOP - 1][1][c][tn];
-        }
+    for (int c = 0; c < 3; ++c) {
+      if (DO_K) {
+        bK[c] = carK[c] + sK[1][0][c] + xch[xb][0][0][c][tn];
+        carK[c] = sK[1][1][c] + xch[xb][0][1][c][tn];
       }
-      if (out_ok) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const bool fx = (fnode >> c) & 1;
-          if (DO_K) {
-            const T val = fx ? T(0) : bK[c];
-            if (MODE == EPI_PLAIN) {
-              yK[3u * nn + c] = val;
-              if (FUSE) dot += (double)val * (double)xin[c];
-            } else if (MODE == EPI_RESID) {
-              const T r = fx ? T(0) : eb[c] - val;
-              ep.r[3u * nn + c] = r;
-              ep.dnew[3u * nn + c] = ep.c2 * ed[c] * r;
-            } else if (MODE == EPI_CHEB) {
-              ep.xs[3u * nn + c] += xin[c];
-              const T r = er[c] - val;
-              ep.r[3u * nn + c] = r;
-              ep.dnew[3u * nn + c] = ep.c1 * xin[c] + ep.c2 * ed[c] * r;
-            } else {                                            // EPI_CHEB0
-              ep.xs[3u * nn + c] = xin[c];
-              const T r = fx ? T(0) : er[c] - val;
-              ep.r[3u * nn + c] = r;
-              ep.dnew[3u * nn + c] = ep.c1 * xin[c] + ep.c2 * ed[c] * r;
-            }
-          }
-          if (DO_M) yM[3u * nn + c] = fx ? T(0) : bM[c];
-        }
+      if (DO_M) {
+        bM[c] = carM[c] + sM[1][0][c] + xch[xb][NOP - 1][0][c][tn];
+        carM[c] = sM[1][1][c] + xch[xb][NOP - 1][1][c][tn];
       }
     }
-    if (L + 2 <= ze) store(s0);                                 // plane L + 2 replaces plane L
-    __syncthreads();
+    T vK[3], vM[3], o1[3], o2[3], o3[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool fx = (cur.fnode >> c) & 1;
+      const T di = MODE != EPI_PLAIN ? cur.di[c] : T(0);
+      if (DO_K) {
+        const T val = fx ? T(0) : bK[c];
+        vK[c] = val;
+        if (MODE == EPI_PLAIN) {
+          if (FUSE && out_ok) dot += (double)val * (double)xin[c];
+        } else if (MODE == EPI_RESID) {
+          const T r = fx ? T(0) : cur.a[c] - val;
+          o1[c] = r;
+          o2[c] = ep.c2 * di * r;
+        } else if (MODE == EPI_CHEB) {
+          o3[c] = cur.xs[c] + xin[c];
+          const T r = cur.a[c] - val;
+          o1[c] = r;
+          o2[c] = ep.c1 * xin[c] + ep.c2 * di * r;
+        } else {                                                // EPI_CHEB0
+          o3[c] = xin[c];
+          const T r = fx ? T(0) : cur.a[c] - val;
+          o1[c] = r;
+          o2[c] = ep.c1 * xin[c] + ep.c2 * di * r;
+        }
+      }
+      if (DO_M) vM[c] = fx ? T(0) : bM[c];
+    }
+    if (DO_K) {
+      if (MODE == EPI_PLAIN) {
+        st3(out_ok ? yK + 3u * nn : nullptr, sink, vK);
+      } else {
+        st3(out_ok ? ep.r + 3u * nn : nullptr, sink, o1);
+        st3(out_ok ? ep.dnew + 3u * nn : nullptr, sink, o2);
+        if (MODE == EPI_CHEB || MODE == EPI_CHEB0) st3(out_ok ? ep.xs + 3u * nn : nullptr, sink, o3);
+      }
+    }
+    if (DO_M) st3(out_ok ? yM + 3u * nn : nullptr, sink, vM);
+    cur = nxt;
+    if (ONEBAR) {
+      const int s_old = sa;
+      sa = sb; sb = sc; sc = s_old;
+      xb ^= 1;
+    } else {
+      const int s_old = sa;
+      sa = sb; sb = s_old;
+      __syncthreads();
+    }
   }
   if (FUSE) {
     const double v = block_sum(dot, sh);
