@@ -48,6 +48,35 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return t;
 }
 
+// Block partial sums for a finishing kernel.  No grid-wide fence is needed:
+// the kernel boundary orders the partials before the kernel that sums them.
+// (A device-scope fence in every block is costly here: on this multi-XCD part
+// it writes back and invalidates the L2 each time.)
+template <int NV>
+__device__ __forceinline__ void write_partials(const double* vals, double* partials, int bid) {
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) partials[(int64_t)bid * NV + v] = vals[v];
+  }
+}
+
+// Fixed-order sum of nb block partials (stride NV) by one block: thread t sums
+// blocks t, t + blockDim.x, ... in order, then block_sum.  Bitwise reproducible.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* partials, int nb, double* out,
+                                             double* sh) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partials[(int64_t)b * NV + v];
+    out[v] = block_sum(t, sh);
+  }
+}
+
+// Finishing kernel of a fused K p . p product: scal[col].pq = sum of the nb
+// partials of column col (partials + col * nb); grid (1, ncols), 256 threads.
+int launch_pq_finish(const DotFuse& fuse, int nb, int ncols, hipStream_t st);
+
 // "Last block finalises" grid reduction of NV values per block.  Each block
 // passes its block sums in `vals` (thread 0 only reads them); partial sums go to
 // partials[slot_base + blockIdx.x*NV + v].  Returns true in every thread of the

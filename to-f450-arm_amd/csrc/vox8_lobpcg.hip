@@ -422,7 +422,6 @@ k_residual(int64_t n, double* __restrict__ S, const double* __restrict__ AS,
            const double* __restrict__ dinv, double* __restrict__ partials,
            unsigned int* __restrict__ counter, double* __restrict__ res_out) {
   __shared__ double lam[K], sh[8], bs[K];
-  __shared__ int s_last;
   const int tid = threadIdx.x;
   if (tid < K) lam[tid] = lamg[tid];
   __syncthreads();
@@ -445,25 +444,20 @@ k_residual(int64_t n, double* __restrict__ S, const double* __restrict__ AS,
     if (tid == 0) bs[j] = t;
   }
   __syncthreads();
-  if (tid == 0) {
-    for (int j = 0; j < K; ++j)
-      __hip_atomic_store(&partials[(int64_t)blockIdx.x * 12 + j], bs[j], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    s_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  for (int j = 0; j < K; ++j) {
+  if (tid == 0)
+    for (int j = 0; j < K; ++j) partials[(int64_t)blockIdx.x * 12 + j] = bs[j];
+}
+
+// Residual norms from the k_residual partials (fixed order; one block).
+__global__ void __launch_bounds__(256)
+k_residual_finish(int k, int nb, const double* __restrict__ partials, double* __restrict__ res_out) {
+  __shared__ double sh[8];
+  for (int j = 0; j < k; ++j) {
     double t = 0.0;
-    for (int b = tid; b < (int)gridDim.x; b += blockDim.x)
-      t += __hip_atomic_load(&partials[(int64_t)b * 12 + j], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partials[(int64_t)b * 12 + j];
     t = block_sum(t, sh);
-    if (tid == 0) res_out[j] = t;
+    if (threadIdx.x == 0) res_out[j] = t;
   }
-  if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void k_copy_cols(int64_t n, int k, const double* __restrict__ a, double* __restrict__ b) {
@@ -551,9 +545,12 @@ __global__ void k_zero_int(int n, int* __restrict__ a) {
 
 template <int K>
 static void launch_residual(const LobpcgWork& w, const double* lam, const double* dinv, hipStream_t st) {
-  hipLaunchKernelGGL((k_residual<K>), dim3(grid_blocks(w.n)), dim3(256), 0, st, w.n, w.S,
+  const unsigned nb = grid_blocks(w.n);
+  hipLaunchKernelGGL((k_residual<K>), dim3(nb), dim3(256), 0, st, w.n, w.S,
                      (const double*)w.AS, (const double*)w.BS, lam, dinv, w.res_partials, w.counters,
                      w.res);
+  hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(256), 0, st, K, (int)nb,
+                     (const double*)w.res_partials, w.res);
 }
 
 #define VOX8_K_SWITCH(k, F, ...)                                                              \

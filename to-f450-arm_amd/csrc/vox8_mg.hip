@@ -273,9 +273,15 @@ k_pow_scale(int64_t n, const float* __restrict__ dinv, float* __restrict__ w, do
     s += (double)v * v;
   }
   const double t = block_sum(s, sh);
+  write_partials<1>(&t, part, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256)
+k_pow_finish(int nb, const double* __restrict__ part, double* __restrict__ scal) {
+  __shared__ double sh[8];
   double tot;
-  if (grid_reduce_last<1>(&t, part, counter, gridDim.x, blockIdx.x, &tot, sh))
-    if (threadIdx.x == 0) { scal[0] = tot > 0 ? 1.0 / sqrt(tot) : 0.0; scal[1] = sqrt(tot); }
+  sum_partials<1>(part, nb, &tot, sh);
+  if (threadIdx.x == 0) { scal[0] = tot > 0 ? 1.0 / sqrt(tot) : 0.0; scal[1] = sqrt(tot); }
 }
 
 __global__ void k_pow_norm(int64_t n, const double* __restrict__ scal, const float* __restrict__ w,
@@ -437,8 +443,11 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
     }
     for (int it = 0; it < steps; ++it) {
       VOX8_TRY(level_apply(L, L.pv, L.r, 1, st, Epilogue<float>()));
-      hipLaunchKernelGGL(k_pow_scale, dim3(nb(n, 1024)), dim3(256), 0, st, n, (const float*)L.dinv, L.r,
+      const unsigned pb = nb(n, 1024);
+      hipLaunchKernelGGL(k_pow_scale, dim3(pb), dim3(256), 0, st, n, (const float*)L.dinv, L.r,
                          mg.red, mg.counter, mg.scal);
+      hipLaunchKernelGGL(k_pow_finish, dim3(1), dim3(256), 0, st, (int)pb, (const double*)mg.red,
+                         mg.scal);
       hipLaunchKernelGGL(k_pow_norm, dim3(nb(n)), dim3(256), 0, st, n, (const double*)mg.scal,
                          (const float*)L.r, L.pv);
     }

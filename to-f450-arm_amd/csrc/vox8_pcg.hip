@@ -142,17 +142,28 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
     rr += (double)rn * (double)rn;
     if (RZ) rz += (double)rn * (double)dinv[i] * (double)rn;
   }
-  double v[2] = {block_sum(rr, sh), RZ ? block_sum(rz, sh) : 0.0}, tot[2];
-  if (grid_reduce_last<2>(v, partials + (int64_t)col * gridDim.x * 2, counters + col, gridDim.x,
-                          blockIdx.x, tot, sh)) {
-    if (threadIdx.x == 0) {
-      scal[col].rr = tot[0];
-      if (RZ) {
-        scal[col].rz_prev = s.rz;
-        scal[col].rz = tot[1];
-      }
-      scal[col].iters = s.iters + 1;
+  const double v[2] = {block_sum(rr, sh), RZ ? block_sum(rz, sh) : 0.0};
+  write_partials<2>(v, partials + (int64_t)col * gridDim.x * 2, blockIdx.x);
+}
+
+// After k_pcg_step: ||r||^2 (and r.D^-1 r) from the partials, iteration count.
+template <bool RZ>
+__global__ void __launch_bounds__(256)
+k_pcg_step_finish(PcgScalars* __restrict__ scal, int maxiter, const double* __restrict__ partials,
+                  int nb) {
+  __shared__ double sh[8];
+  const int col = blockIdx.y;
+  const PcgScalars s = scal[col];
+  if (s.done || s.iters >= maxiter) return;
+  double tot[2];
+  sum_partials<2>(partials + (int64_t)col * nb * 2, nb, tot, sh);
+  if (threadIdx.x == 0) {
+    scal[col].rr = tot[0];
+    if (RZ) {
+      scal[col].rz_prev = s.rz;
+      scal[col].rz = tot[1];
     }
+    scal[col].iters = s.iters + 1;
   }
 }
 
@@ -186,14 +197,42 @@ k_pcg_rz(int64_t n_dofs, const T* __restrict__ r, const T* __restrict__ z, int64
     for (int u = 0; u < 4; ++u) rz += (double)rv[u] * (double)zv[u];
   }
   for (; i < n_dofs; i += stride) rz += (double)r[i] * (double)z[i];
-  double v = block_sum(rz, sh), tot;
-  if (grid_reduce_last<1>(&v, partials + (int64_t)col * gridDim.x, counters + col, gridDim.x,
-                          blockIdx.x, &tot, sh)) {
-    if (threadIdx.x == 0) {
-      scal[col].rz_prev = s.rz;
-      scal[col].rz = tot;
-    }
+  const double v = block_sum(rz, sh);
+  write_partials<1>(&v, partials + (int64_t)col * gridDim.x, blockIdx.x);
+}
+
+// After k_pcg_rz: rho = r.z from the partials.
+__global__ void __launch_bounds__(256)
+k_pcg_rz_finish(PcgScalars* __restrict__ scal, int maxiter, const double* __restrict__ partials,
+                int nb) {
+  __shared__ double sh[8];
+  const int col = blockIdx.y;
+  const PcgScalars s = scal[col];
+  if (s.done || s.iters >= maxiter) return;
+  double tot;
+  sum_partials<1>(partials + (int64_t)col * nb, nb, &tot, sh);
+  if (threadIdx.x == 0) {
+    scal[col].rz_prev = s.rz;
+    scal[col].rz = tot;
   }
+}
+
+// After a fused K p product: p.q from the partials.
+__global__ void __launch_bounds__(256)
+k_pq_finish(PcgScalars* __restrict__ scal, const double* __restrict__ partials, int nb) {
+  __shared__ double sh[8];
+  const int col = blockIdx.y;
+  if (scal[col].done) return;
+  double tot;
+  sum_partials<1>(partials + (int64_t)col * nb, nb, &tot, sh);
+  if (threadIdx.x == 0) scal[col].pq = tot;
+}
+
+int launch_pq_finish(const DotFuse& fuse, int nb, int ncols, hipStream_t st) {
+  hipLaunchKernelGGL(k_pq_finish, dim3(1, ncols), dim3(256), 0, st, fuse.scal,
+                     (const double*)fuse.partials, nb);
+  VOX8_HIP(hipGetLastError());
+  return OK;
 }
 
 template <typename T>
@@ -273,16 +312,22 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
         VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
         hipLaunchKernelGGL((k_pcg_step<T, true>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
                            w.scal, maxiter, w.partials, w.counters);
+        hipLaunchKernelGGL(k_pcg_step_finish<true>, dim3(1, nrhs), blk, 0, st, w.scal, maxiter,
+                           (const double*)w.partials, (int)nb);
       } else {
         hipLaunchKernelGGL(k_pcg_check, dim3(1, nrhs), dim3(1), 0, st, w.scal, maxiter, w.skip);
         VOX8_TRY((*M)(w.r, w.x, nrhs, st, w.skip));
         hipLaunchKernelGGL((k_pcg_rz<T>), grid, blk, 0, st, n, (const T*)w.r, (const T*)w.x, n,
                            w.scal, maxiter, w.partials, w.counters);
+        hipLaunchKernelGGL(k_pcg_rz_finish, dim3(1, nrhs), blk, 0, st, w.scal, maxiter,
+                           (const double*)w.partials, (int)nb);
         hipLaunchKernelGGL((k_pcg_dir_z<T>), grid, blk, 0, st, n, (const T*)w.x, w.p, n,
                            (const PcgScalars*)w.scal, maxiter);
         VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
         hipLaunchKernelGGL((k_pcg_step<T, false>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
                            w.scal, maxiter, w.partials, w.counters);
+        hipLaunchKernelGGL(k_pcg_step_finish<false>, dim3(1, nrhs), blk, 0, st, w.scal, maxiter,
+                           (const double*)w.partials, (int)nb);
       }
     }
     VOX8_HIP(hipGetLastError());

@@ -100,11 +100,8 @@ k_apply_dense(Grid g, const T* __restrict__ fK, const T* __restrict__ fM,
     if (DO_K && fuse.scal && live) dot += (double)vk * (double)(fx ? T(0) : x[3 * n + c]);
   }
   if (fuse.scal) {
-    double v = block_sum(dot, sh), tot;
-    if (grid_reduce_last<1>(&v, fuse.partials + (int64_t)col * gridDim.x, fuse.counters + col,
-                            gridDim.x, blockIdx.x, &tot, sh)) {
-      if (threadIdx.x == 0) fuse.scal[col].pq = tot;
-    }
+    const double v = block_sum(dot, sh);
+    write_partials<1>(&v, fuse.partials + (int64_t)col * gridDim.x, blockIdx.x);
   }
 }
 
@@ -184,6 +181,7 @@ int launch_apply_dense(const Grid& g, int op, const T* fK, const T* fM, const T*
     hipLaunchKernelGGL((k_apply_dense<T, true, true>), grid, dim3(B), 0, st, g, fK, fM, Ke, Me,
                        fixed, x, ldx, yK, yM, ldy, fuse);
   VOX8_HIP(hipGetLastError());
+  if (fuse.scal) VOX8_TRY(launch_pq_finish(fuse, (int)grid.x, nrhs, st));
   return OK;
 }
 template int launch_apply_dense<double>(const Grid&, int, const double*, const double*,

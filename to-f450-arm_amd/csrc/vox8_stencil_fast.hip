@@ -109,23 +109,62 @@ __device__ __forceinline__ void inv8(const T m[8], T v[8]) {
   v[4] = s2 - d2; v[5] = s2 + d2; v[6] = s3 - d3; v[7] = s3 + d3;
 }
 
-// A node's three components as one access.
-template <typename T>
-struct Vec3 {
-  T a, b, c;
-  __device__ __forceinline__ T operator[](int i) const { return i == 0 ? a : (i == 1 ? b : c); }
-};
-template <typename T>
-__device__ __forceinline__ Vec3<T> ld3(const T* __restrict__ p, uint32_t node) {
-  return *reinterpret_cast<const Vec3<T>*>(p + 3u * node);
+// Buffer-resource access (raw buffer, byte offsets).  DWORD3 of the
+// descriptor is the gfx9-family default for untyped 32-bit access.  Loads use
+// only the 8- and 32-bit builtins (the compiler merges adjacent dwords into
+// dwordx3 / dwordx4 + dwordx2 loads; element extraction from the results of
+// the wider load builtins is miscompiled by this toolchain).
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t OOB = 0x80000000u;   // past every array (all < 2^31 bytes)
+__device__ __forceinline__ Rsrc make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
 }
-// Lanes without an owned node write to a sink instead of skipping the store,
-// so the stores are not in a branch (see the loop).
-__device__ double store_sink[4];
+__device__ __forceinline__ uint32_t ld_u8(Rsrc r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+__device__ __forceinline__ uint32_t ld_w(Rsrc r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+__device__ __forceinline__ void st_w(Rsrc r, uint32_t off, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, 0);
+}
+__device__ __forceinline__ float ld1(Rsrc r, uint32_t off, float) {
+  return __builtin_bit_cast(float, ld_w(r, off));
+}
+__device__ __forceinline__ double ld1(Rsrc r, uint32_t off, double) {
+  return __builtin_bit_cast(double, ((uint64_t)ld_w(r, off + 4u) << 32) | ld_w(r, off));
+}
+__device__ __forceinline__ void st1(Rsrc r, uint32_t off, float v) {
+  st_w(r, off, __builtin_bit_cast(uint32_t, v));
+}
+__device__ __forceinline__ void st1(Rsrc r, uint32_t off, double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  st_w(r, off, (uint32_t)u);
+  st_w(r, off + 4u, (uint32_t)(u >> 32));
+}
+// a node's three components
 template <typename T>
-__device__ __forceinline__ void st3(T* dst, Vec3<T>* sink, const T v[3]) {
-  Vec3<T>* p = dst ? reinterpret_cast<Vec3<T>*>(dst) : sink;
-  *p = Vec3<T>{v[0], v[1], v[2]};
+__device__ __forceinline__ void ld3(Rsrc r, uint32_t off, T v[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = ld1(r, off + (uint32_t)(c * sizeof(T)), T(0));
+}
+// stores of a triple as one 96-bit (FP32) / 128 + 64-bit (FP64) access built
+// from a vector value (the compiler does not always merge dword stores)
+__device__ __forceinline__ void st3(Rsrc r, uint32_t off, const float v[3]) {
+  typedef unsigned int u3 __attribute__((ext_vector_type(3)));
+  const u3 w = {__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
+                __builtin_bit_cast(uint32_t, v[2])};
+  __builtin_amdgcn_raw_buffer_store_b96(w, r, off, 0, 0);
+}
+__device__ __forceinline__ void st3(Rsrc r, uint32_t off, const double v[3]) {
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+  const uint64_t a = __builtin_bit_cast(uint64_t, v[0]), b = __builtin_bit_cast(uint64_t, v[1]),
+                 c = __builtin_bit_cast(uint64_t, v[2]);
+  const u4 w = {(uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, (uint32_t)(b >> 32)};
+  const u2 u = {(uint32_t)c, (uint32_t)(c >> 32)};
+  __builtin_amdgcn_raw_buffer_store_b128(w, r, off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b64(u, r, off + 16u, 0, 0);
 }
 
 // Value of lane + 1 of the wavefront (DPP wave_shl:1, one VALU move per
@@ -296,9 +335,32 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const int ex = ex0 + tx, ey = ey0 + ty;                      // element
   const bool exy_in = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
 
-  // plane staging: thread loads nodes q = t + i NT of the (PX x PY) plane.
-  // Offsets are 32-bit (the launcher checks 3 n_nodes < 2^31); the in-plane
-  // part is fixed per thread, the plane part advances by NX NY per layer.
+  // Memory operations go through buffer resources (base and byte size in
+  // scalar registers): the hardware range check makes a load at offset OOB
+  // return 0 and drops a store there, so no global load or store sits in a
+  // branch and lanes without an owned node need no separate path.  (The memory
+  // counter covers loads and stores in issue order; with path-dependent counts
+  // the compiler's waits for a load would also wait for just-issued stores.)
+  // Byte offsets are 32-bit: the launcher checks every array < 2^31 bytes.
+  // Every load of the loop is issued one layer before its use: at the top of
+  // layer L the element/epilogue operands of layer L + 1 and the node plane
+  // L + 3 (one barrier) or L + 2 (two barriers).
+  const uint32_t vbytes = 3u * (uint32_t)g.n_nodes * (uint32_t)sizeof(T);
+  const uint32_t ebytes = (uint32_t)g.n_elem * (uint32_t)sizeof(T);
+  const Rsrc rx = make_rsrc(x, vbytes);
+  const Rsrc rfix = make_rsrc(fixed, (uint32_t)g.n_nodes);
+  const Rsrc rfk = make_rsrc(fK, DO_K ? ebytes : 0u);
+  const Rsrc rfm = make_rsrc(fM, DO_M ? ebytes : 0u);
+  const Rsrc rdi = make_rsrc(ep.dinv, MODE != EPI_PLAIN ? vbytes : 0u);
+  const Rsrc rb = make_rsrc(MODE == EPI_RESID ? ep.b : (MODE == EPI_CHEB ? ep.r : x),
+                            MODE == EPI_PLAIN ? 0u : vbytes);        // epilogue input a
+  const Rsrc rxs = make_rsrc(ep.xs, (MODE == EPI_CHEB || MODE == EPI_CHEB0) ? vbytes : 0u);
+  const Rsrc ry1 = make_rsrc(MODE == EPI_PLAIN ? yK : ep.r, DO_K ? vbytes : 0u);
+  const Rsrc ry2 = make_rsrc(ep.dnew, MODE != EPI_PLAIN ? vbytes : 0u);
+  const Rsrc rym = make_rsrc(yM, DO_M ? vbytes : 0u);
+
+  // plane staging: thread loads nodes q = t + i NT of the (PX x PY) plane;
+  // the in-plane part of the node index is fixed per thread
   const uint32_t plane = (uint32_t)g.NX * (uint32_t)g.NY;
   uint32_t qoff[NLD];
   bool qin[NLD];
@@ -310,31 +372,19 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     qin[i] = q < PN && nxg >= 0 && nxg < g.NX && nyg >= 0 && nyg < g.NY;
     qoff[i] = qin[i] ? (uint32_t)nxg + (uint32_t)g.NX * (uint32_t)nyg : 0u;
   }
-  // Memory operations of the loop.  Every global load is issued one layer
-  // before its use: at the top of layer L the node plane L + 2 and the
-  // element/epilogue operands of layer L + 1.  No global load or store sits in
-  // a branch: out-of-range loads read node/element 0 and their value is
-  // discarded, and lanes that own no node store to a sink.  (The memory
-  // counter covers loads and stores in issue order; with path-dependent counts
-  // the compiler's waits for a load would also wait for just-issued stores.)
-  // Node triples move as one 3-dword (FP32) / 3-qword (FP64) access per lane.
   T pre[NLD][3];
   T prd[NLD][3];                                               // EPI_CHEB0: D^-1 of the plane
   uint32_t prf[NLD];
-  bool pin[NLD];
   auto fetch = [&](int iz) {
     const bool zin = iz >= 0 && iz < g.NZ;
 #pragma unroll
     for (int i = 0; i < NLD; ++i) {
-      pin[i] = zin && qin[i];
-      const uint32_t n = pin[i] ? qoff[i] + plane * (uint32_t)iz : 0u;
-      prf[i] = fixed[n];
-      const Vec3<T> v = ld3(x, n);
-      pre[i][0] = v.a; pre[i][1] = v.b; pre[i][2] = v.c;
-      if (MODE == EPI_CHEB0) {
-        const Vec3<T> d = ld3(ep.dinv, n);
-        prd[i][0] = d.a; prd[i][1] = d.b; prd[i][2] = d.c;
-      }
+      const bool in = zin && qin[i];
+      const uint32_t n = qoff[i] + plane * (uint32_t)iz;
+      const uint32_t o3 = in ? 3u * (uint32_t)sizeof(T) * n : OOB, o1 = in ? n : OOB;
+      prf[i] = ld_u8(rfix, o1);
+      ld3(rx, o3, pre[i]);
+      if (MODE == EPI_CHEB0) ld3(rdi, o3, prd[i]);
     }
   };
   auto store = [&](int slot) {
@@ -346,7 +396,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
         for (int c = 0; c < 3; ++c) {
           T v = pre[i][c];
           if (MODE == EPI_CHEB0) v *= ep.c0 * prd[i][c];
-          pl[slot][c][q] = (pin[i] && !((prf[i] >> c) & 1u)) ? v : T(0);
+          pl[slot][c][q] = ((prf[i] >> c) & 1u) ? T(0) : v;   // out of range: v = 0
         }
     }
   };
@@ -354,40 +404,58 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const uint32_t eplane = (uint32_t)g.nx * (uint32_t)g.ny;
   const uint32_t nxy = own_in ? (uint32_t)gx + (uint32_t)g.NX * (uint32_t)gy : 0u;
 
-  // operands of one layer: element factors of element layer L and, for the
-  // owned node of node plane L, its mask and the epilogue inputs
-  struct LayerOps {
-    T fk, fm;
-    uint32_t fnode;
-    Vec3<T> a;    // RESID: b; CHEB: r; CHEB0: b (= x)
-    Vec3<T> di;   // D^-1
-    Vec3<T> xs;   // CHEB: iterate
+  // Element factors of element layer L + 1 are loaded during layer L.  The
+  // operands of the owned node of plane L (mask, epilogue inputs, and the
+  // byte offset the epilogue stores to: OOB when the layer owns no node) are
+  // loaded once the element is computed (so they are not live during it) and
+  // used after the barrier.
+  struct NodeOps {
+    uint32_t fnode, off;
+    T a[3];    // RESID: b; CHEB: r; CHEB0: b (= x)
+    T di[3];   // D^-1
+    T xs[3];   // CHEB: iterate
   };
-  auto load_ops = [&](int L, LayerOps& o) {
+  auto load_fac = [&](int L, T& fk, T& fm) {
     const bool ein = exy_in && L >= 0 && L < g.nz;
-    const uint32_t e = ein ? exy + eplane * (uint32_t)L : 0u;
-    if (DO_K) o.fk = fK[e];
-    if (DO_M) o.fm = fM[e];
-    const bool ok = own_in && L >= zs && L < ze;
-    const uint32_t nn = ok ? nxy + plane * (uint32_t)L : 0u;
-    o.fnode = fixed[nn];
-    if (MODE == EPI_RESID) o.a = ld3(ep.b, nn);
-    if (MODE == EPI_CHEB) o.a = ld3(ep.r, nn);
-    if (MODE == EPI_CHEB0) o.a = ld3(x, nn);
-    if (MODE != EPI_PLAIN) o.di = ld3(ep.dinv, nn);
-    if (MODE == EPI_CHEB) o.xs = ld3(ep.xs, nn);
+    const uint32_t oe = ein ? (exy + eplane * (uint32_t)L) * (uint32_t)sizeof(T) : OOB;
+    if (DO_K) fk = ld1(rfk, oe, T(0));
+    if (DO_M) fm = ld1(rfm, oe, T(0));
   };
-  Vec3<T>* const sink = reinterpret_cast<Vec3<T>*>(store_sink);
+  auto load_node = [&](int L, NodeOps& o) {
+    const bool ok = own_in && L >= zs && L < ze;
+    const uint32_t nn = nxy + plane * (uint32_t)L;
+    o.off = ok ? 3u * (uint32_t)sizeof(T) * nn : OOB;
+    o.fnode = ld_u8(rfix, ok ? nn : OOB);
+    if (MODE != EPI_PLAIN) {
+      ld3(rb, o.off, o.a);
+      ld3(rdi, o.off, o.di);
+    }
+    if (MODE == EPI_CHEB) ld3(rxs, o.off, o.xs);
+  };
+  // Hadamard coefficients of the x-y face of the element at (tx, ty) in a
+  // node plane: per component [s+, s-, d+, d-] with s/d the x sum/difference
+  // of rows ty and ty + 1 combined in y.  The top face of element layer L is
+  // the bottom face of layer L + 1 and is carried, so each layer transforms
+  // one face.
+  auto face = [&](int slot, T fc[3][4]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T v0 = pl[slot][c][ty * PX + tx], v1 = pl[slot][c][ty * PX + tx + 1];
+      const T v2 = pl[slot][c][(ty + 1) * PX + tx], v3 = pl[slot][c][(ty + 1) * PX + tx + 1];
+      const T s0 = v1 + v0, d0 = v1 - v0, s1 = v3 + v2, d1 = v3 - v2;
+      fc[c][0] = s1 + s0; fc[c][1] = s1 - s0; fc[c][2] = d1 + d0; fc[c][3] = d1 - d0;
+    }
+  };
 
   T carK[3] = {0, 0, 0}, carM[3] = {0, 0, 0};
   double dot = 0.0;
-  LayerOps cur, nxt;
+  T fk = T(0), fm = T(0), fk1 = T(0), fm1 = T(0);
   fetch(zs - 1);
   store(0);
   fetch(zs);
   store(1);
   if (ONEBAR) fetch(zs + 1);
-  load_ops(zs - 1, cur);
+  load_fac(zs - 1, fk, fm);
   // All loads of the prologue complete here, so none is pending at the loop
   // header.
   __builtin_amdgcn_s_waitcnt(0x0F70);                          // vmcnt(0)
@@ -395,30 +463,33 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const int tn = owner ? t + TX : t;                           // element (tx, ty + 1)
   // slots of planes L (sa), L + 1 (sb) and, one barrier: L + 2 (sc)
   int sa = 0, sb = 1, sc = 2, xb = 0;
+  T fbot[3][4];                                                // bottom face of layer L
+  face(sa, fbot);
   for (int L = zs - 1; L < ze; ++L) {
     // One barrier: plane L + 2 (loaded during layer L - 1) goes to the slot of
     // plane L - 1, which every thread finished reading before the previous
     // barrier; it is read after this layer's barrier.
     if (ONEBAR) store(sc);
-    // unconditional (past the chunk the addresses fall back to 0 and the
-    // values are unused)
-    load_ops(L + 1, nxt);
+    load_fac(L + 1, fk1, fm1);
     fetch(ONEBAR ? L + 3 : L + 2);
-    const bool ein = exy_in && L >= 0 && L < g.nz;
-    const T fk = (DO_K && ein) ? cur.fk : T(0);
-    const T fm = (DO_M && ein) ? cur.fm : T(0);
-    const bool out_ok = own_in && L >= zs;
-    const uint32_t nn = nxy + plane * (uint32_t)(L < 0 ? 0 : L);
     T mono[3][8];
+    {
+      T ftop[3][4];
+      face(sb, ftop);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      T v[8];
+      for (int c = 0; c < 3; ++c) {
+        // z stage: monomials [1, xi, eta, zeta, xi eta, eta zeta, xi zeta, xi eta zeta]
+        mono[c][0] = ftop[c][0] + fbot[c][0];
+        mono[c][3] = ftop[c][0] - fbot[c][0];
+        mono[c][2] = ftop[c][1] + fbot[c][1];
+        mono[c][5] = ftop[c][1] - fbot[c][1];
+        mono[c][1] = ftop[c][2] + fbot[c][2];
+        mono[c][6] = ftop[c][2] - fbot[c][2];
+        mono[c][4] = ftop[c][3] + fbot[c][3];
+        mono[c][7] = ftop[c][3] - fbot[c][3];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int ox = k & 1, oy = (k >> 1) & 1, oz = k >> 2;
-        v[k] = pl[oz ? sb : sa][c][(ty + oy) * PX + tx + ox];
+        for (int k = 0; k < 4; ++k) fbot[c][k] = ftop[c][k];
       }
-      fwd8(v, mono[c]);
     }
     T xin[3];                                                   // owned node's input value
 #pragma unroll
@@ -426,7 +497,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     T sK[2][2][3], sM[2][2][3];
     if (DO_K) {
       T f[3][8];
-      element_K<T, PAT>(C, fk, mono, f);
+      element_K<T, PAT>(C, fk, mono, f);                       // fk = 0 outside the grid
       x_exchange(f, sK);
 #pragma unroll
       for (int oz = 0; oz < 2; ++oz)
@@ -442,6 +513,8 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
 #pragma unroll
         for (int c = 0; c < 3; ++c) xch[xb][NOP - 1][oz][c][t] = sM[0][oz][c];
     }
+    NodeOps cur;
+    load_node(L, cur);
     __syncthreads();
     if (!ONEBAR) store(sa);                                     // plane L + 2 replaces plane L
     T bK[3], bM[3];
@@ -460,41 +533,41 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const bool fx = (cur.fnode >> c) & 1;
-      const T di = MODE != EPI_PLAIN ? cur.di[c] : T(0);
       if (DO_K) {
         const T val = fx ? T(0) : bK[c];
         vK[c] = val;
         if (MODE == EPI_PLAIN) {
-          if (FUSE && out_ok) dot += (double)val * (double)xin[c];
+          if (FUSE) dot += cur.off != OOB ? (double)val * (double)xin[c] : 0.0;
         } else if (MODE == EPI_RESID) {
           const T r = fx ? T(0) : cur.a[c] - val;
           o1[c] = r;
-          o2[c] = ep.c2 * di * r;
+          o2[c] = ep.c2 * cur.di[c] * r;
         } else if (MODE == EPI_CHEB) {
           o3[c] = cur.xs[c] + xin[c];
           const T r = cur.a[c] - val;
           o1[c] = r;
-          o2[c] = ep.c1 * xin[c] + ep.c2 * di * r;
+          o2[c] = ep.c1 * xin[c] + ep.c2 * cur.di[c] * r;
         } else {                                                // EPI_CHEB0
           o3[c] = xin[c];
           const T r = fx ? T(0) : cur.a[c] - val;
           o1[c] = r;
-          o2[c] = ep.c1 * xin[c] + ep.c2 * di * r;
+          o2[c] = ep.c1 * xin[c] + ep.c2 * cur.di[c] * r;
         }
       }
       if (DO_M) vM[c] = fx ? T(0) : bM[c];
     }
     if (DO_K) {
       if (MODE == EPI_PLAIN) {
-        st3(out_ok ? yK + 3u * nn : nullptr, sink, vK);
+        st3(ry1, cur.off, vK);
       } else {
-        st3(out_ok ? ep.r + 3u * nn : nullptr, sink, o1);
-        st3(out_ok ? ep.dnew + 3u * nn : nullptr, sink, o2);
-        if (MODE == EPI_CHEB || MODE == EPI_CHEB0) st3(out_ok ? ep.xs + 3u * nn : nullptr, sink, o3);
+        st3(ry1, cur.off, o1);
+        st3(ry2, cur.off, o2);
+        if (MODE == EPI_CHEB || MODE == EPI_CHEB0) st3(rxs, cur.off, o3);
       }
     }
-    if (DO_M) st3(out_ok ? yM + 3u * nn : nullptr, sink, vM);
-    cur = nxt;
+    if (DO_M) st3(rym, cur.off, vM);
+    fk = fk1;
+    fm = fm1;
     if (ONEBAR) {
       const int s_old = sa;
       sa = sb; sb = sc; sc = s_old;
@@ -507,11 +580,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   }
   if (FUSE) {
     const double v = block_sum(dot, sh);
-    double tot;
-    if (grid_reduce_last<1>(&v, fuse.partials + (int64_t)col * ntiles, fuse.counters + col, ntiles, tile,
-                            &tot, sh)) {
-      if (threadIdx.x == 0) fuse.scal[col].pq = tot;
-    }
+    write_partials<1>(&v, fuse.partials + (int64_t)col * ntiles, tile);
   }
 }
 
@@ -546,8 +615,9 @@ template <typename T>
 int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,
                       const T* fM, const uint8_t* fixed, const T* x, int64_t ldx, T* yK, T* yM,
                       int64_t ldy, int nrhs, hipStream_t st, DotFuse fuse, Epilogue<T> ep) {
-  if (3 * g.n_nodes >= ((int64_t)1 << 31)) {
-    set_error("structured stencil: grid too large for 32-bit offsets (use the dense kernel)");
+  if (3 * g.n_nodes * (int64_t)sizeof(T) >= ((int64_t)1 << 31) ||
+      g.n_elem * (int64_t)sizeof(T) >= ((int64_t)1 << 31)) {
+    set_error("structured stencil: grid too large for 32-bit byte offsets (use the dense kernel)");
     return ERR_ARG;
   }
   const int zc = fast_zchunk(g);
@@ -575,6 +645,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
 #undef VOX8_PAT
 #undef VOX8_LAUNCH
   VOX8_HIP(hipGetLastError());
+  if (fz) VOX8_TRY(launch_pq_finish(fuse, fast_blocks(g), nrhs, st));
   return OK;
 }
 
