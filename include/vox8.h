@@ -126,6 +126,12 @@ int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, 
  * nu 2, alpha 12, coarse_dofs 700, cycle 2, w_min_dofs 50000. */
 int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, int cycle,
                         int64_t w_min_dofs);
+/* Replay each multigrid cycle as a captured HIP graph (default 1) or launch
+ * its kernels one by one (0).  A cycle is captured once per (r, z, nrhs,
+ * skip mask) and hierarchy state and replayed afterwards; results are the
+ * same launches in the same order.  No counterpart in the reference (its
+ * solvers are SciPy/CuPy calls). */
+int vox8_set_mg_graphs(vox8_ctx* ctx, int enable);
 
 /* Adjoint quantities for the current density:
  *   d_ce[e]  = sum_j w_j U_j,e^T Ke U_j,e      (elem_compliance, src/fem.py:605-615)

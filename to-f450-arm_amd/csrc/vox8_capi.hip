@@ -166,7 +166,7 @@ int ensure_pcg_work(vox8_ctx* c, PcgWork<T>& w, int ncols) {
   const int64_t per_col = apply_blocks > 3 * 1024 ? apply_blocks : 3 * 1024;
   VOX8_TRY(dalloc(c, &w.partials, (size_t)(ncols * per_col)));
   VOX8_TRY(dalloc(c, &w.counters, ncols));
-  VOX8_HIP(hipMemset(w.counters, 0, sizeof(unsigned int) * ncols));
+  VOX8_HIP(hipMemsetAsync(w.counters, 0, sizeof(unsigned int) * ncols, c->st));
   VOX8_TRY(halloc(c, &w.host_scal, ncols));
   VOX8_TRY(halloc(c, &w.host_scale, ncols));
   w.ncols = ncols;
@@ -194,7 +194,7 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   VOX8_TRY(dalloc(c, &w.res, 16));
   VOX8_TRY(dalloc(c, &w.res_partials, (size_t)1024 * 12));
   VOX8_TRY(dalloc(c, &w.counters, 4));
-  VOX8_HIP(hipMemset(w.counters, 0, sizeof(unsigned int) * 4));
+  VOX8_HIP(hipMemsetAsync(w.counters, 0, sizeof(unsigned int) * 4, c->st));
   VOX8_TRY(halloc(c, &w.host, 32));
   VOX8_TRY(dalloc(c, &w.guard, (size_t)LOBPCG_GUARD_MAX * n));
   VOX8_TRY(dalloc(c, &w.conv, 16));
@@ -225,7 +225,7 @@ int ensure_mg(vox8_ctx* c, int ncols) {
   MGAlloc al{c, mg_dev_alloc, mg_pinned_alloc};
   if (!c->mg.built) {
     VOX8_TRY(mg_build(c->mg, c->g, c->h_active.data(), c->h_fixed.data(), c->Ke, c->pattern, c->c32,
-                      c->fK32, c->d_active, c->d_fixed, al));
+                      c->fK32, c->d_active, c->d_fixed, al, c->st));
     c->mg_version = ~0ull;
   }
   VOX8_TRY(mg_reserve(c->mg, ncols, al));
@@ -296,7 +296,10 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
       (s = dalloc(c, &c->d_sums, 32)) || (s = dalloc(c, &c->d_wts, 32)) ||
       (s = dalloc(c, &c->d_den, ne)) || (s = dalloc(c, &c->d_fmask, ne)))
     return fail(s);
-  if (hipMemset(c->red_counters, 0, 64 * sizeof(unsigned int)) != hipSuccess) {
+  // stream-ordered (a legacy-stream memset conflicts with graph captures in
+  // progress on other threads)
+  if (hipMemsetAsync(c->red_counters, 0, 64 * sizeof(unsigned int), c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess) {
     set_error("hipMemset failed");
     return fail(ERR_HIP);
   }
@@ -415,6 +418,14 @@ int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs, int 
   if (c->mg.coarse_dofs != coarse_dofs) c->mg.built = false;
   c->mg.coarse_dofs = coarse_dofs;
   c->mg_version = ~0ull;
+  ++c->mg.epoch;
+  return OK;
+}
+
+int vox8_set_mg_graphs(vox8_ctx* c, int enable) {
+  VOX8_CHECK(c != nullptr, "null context");
+  c->mg.use_graphs = enable != 0;
+  if (!c->mg.use_graphs) mg_drop_graphs(c->mg);
   return OK;
 }
 

@@ -60,7 +60,7 @@ struct Epilogue {
 };
 
 int fast_zchunk(const Grid& g);
-int fast_blocks(const Grid& g);
+int fast_blocks(const Grid& g, int zc);      // workgroups per column
 
 template <typename T>
 int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,

@@ -24,6 +24,19 @@ struct MGLevel {
   float *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d2 = nullptr;
 };
 
+// One captured cycle: the launches of mg_apply for fixed (r, z, ncols, skip)
+// replayed as a HIP graph (a W-cycle is ~150 launches, most of them tiny
+// coarse-level kernels the host cannot issue as fast as the GPU runs them).
+struct MGGraph {
+  const double* r = nullptr;
+  double* z = nullptr;
+  int ncols = 0;
+  const int* skip = nullptr;
+  uint64_t epoch = 0;
+  uint64_t last_use = 0;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct MultiGrid {
   std::vector<MGLevel> lv;
   int ncols = 0;               // capacity of the work vectors
@@ -44,7 +57,22 @@ struct MultiGrid {
   float* tmp32 = nullptr;      // fine-level input/output staging (ncols * n0)
   bool built = false;
   bool updated = false;
+  // captured cycles; epoch changes whenever a captured launch sequence would
+  // (hierarchy, work vectors, coefficients, options)
+  bool use_graphs = true;
+  uint64_t epoch = 1;
+  uint64_t use_clock = 0;
+  int capture_failures = 0;
+  std::vector<MGGraph> graphs;
+  hipStream_t cap_st = nullptr;   // capture only (the null stream cannot be captured)
+  MultiGrid() = default;
+  MultiGrid(const MultiGrid&) = delete;
+  MultiGrid& operator=(const MultiGrid&) = delete;
+  ~MultiGrid();
 };
+
+// Destroy every captured cycle (device state is unchanged).
+void mg_drop_graphs(MultiGrid& mg);
 
 // Allocation helpers are supplied by the context (device/pinned, tracked).
 struct MGAlloc {
@@ -56,7 +84,7 @@ struct MGAlloc {
 // Build the level hierarchy for the fine grid and its host masks.
 int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8_t* h_fixed,
              const double* Ke, int pattern0, const FastCoef<float>& c0, float* fK0_32,
-             uint8_t* d_active0, uint8_t* d_fixed0, const MGAlloc& al);
+             uint8_t* d_active0, uint8_t* d_fixed0, const MGAlloc& al, hipStream_t st);
 // Per-density setup: restrict factors, Jacobi diagonals, lambda_max, coarse inverse.
 int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t st);
 // Ensure work vectors for ncols columns.

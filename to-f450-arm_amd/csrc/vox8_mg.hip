@@ -19,6 +19,8 @@
 // Chebyshev epilogues; the outer Krylov method stays FP64.
 #include <cmath>
 
+#include <mutex>
+
 #include "vox8_mg.h"
 
 namespace vox8 {
@@ -301,7 +303,7 @@ static int level_apply(const MGLevel& L, const float* x, float* y, int ncols, hi
 
 int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8_t* h_fixed,
              const double* Ke, int pattern0, const FastCoef<float>& c0, float* fK0_32,
-             uint8_t* d_active0, uint8_t* d_fixed0, const MGAlloc& al) {
+             uint8_t* d_active0, uint8_t* d_fixed0, const MGAlloc& al, hipStream_t st) {
   VOX8_CHECK(pattern0 != PAT_DENSE, "multigrid needs the structured (box-element) stencil");
   mg.lv.clear();
   std::vector<uint8_t> act(h_active, h_active + g0.n_elem), fix(h_fixed, h_fixed + g0.n_nodes);
@@ -328,8 +330,9 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
       VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_elem)); L.fK = (float*)p;
       VOX8_TRY(al.dev(al.ctx, &p, g.n_elem)); L.active = (uint8_t*)p;
       VOX8_TRY(al.dev(al.ctx, &p, g.n_nodes)); L.fixed = (uint8_t*)p;
-      VOX8_HIP(hipMemcpy(L.active, act.data(), g.n_elem, hipMemcpyHostToDevice));
-      VOX8_HIP(hipMemcpy(L.fixed, fix.data(), g.n_nodes, hipMemcpyHostToDevice));
+      VOX8_HIP(hipMemcpyAsync(L.active, act.data(), g.n_elem, hipMemcpyHostToDevice, st));
+      VOX8_HIP(hipMemcpyAsync(L.fixed, fix.data(), g.n_nodes, hipMemcpyHostToDevice, st));
+      VOX8_HIP(hipStreamSynchronize(st));   // act / fix change on the next level
     }
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.dinv = (float*)p;
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.pv = (float*)p;
@@ -386,12 +389,13 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
   }
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 6144)); mg.red = (double*)p;
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(unsigned int) * 4)); mg.counter = (unsigned int*)p;
-  VOX8_HIP(hipMemset(mg.counter, 0, sizeof(unsigned int) * 4));
+  VOX8_HIP(hipMemsetAsync(mg.counter, 0, sizeof(unsigned int) * 4, st));
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 8)); mg.scal = (double*)p;
   VOX8_TRY(al.pinned(al.ctx, &p, sizeof(double) * 8)); mg.host = (double*)p;
   mg.ncols = 0;
   mg.built = true;
   mg.updated = false;
+  ++mg.epoch;
   return OK;
 }
 
@@ -407,6 +411,7 @@ int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al) {
     VOX8_TRY(al.dev(al.ctx, &p, bytes)); L.d2 = (float*)p;
   }
   mg.ncols = ncols;
+  ++mg.epoch;
   return OK;
 }
 
@@ -487,6 +492,7 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
     }
   }
   mg.updated = true;
+  ++mg.epoch;
   return OK;
 }
 
@@ -568,8 +574,8 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   return smooth(L, mg.nu, mg.alpha, ncols, st, false, skip);
 }
 
-int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st, const int* skip) {
-  VOX8_CHECK(mg.updated && ncols <= mg.ncols, "multigrid not set up for this density / width");
+static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,
+                             const int* skip) {
   MGLevel& F = mg.lv[0];
   hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b,
                      skip);
@@ -578,6 +584,89 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
   hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
                      (const float*)F.x, z, skip);
   VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+void mg_drop_graphs(MultiGrid& mg) {
+  for (auto& gr : mg.graphs)
+    if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+  mg.graphs.clear();
+}
+
+MultiGrid::~MultiGrid() {
+  mg_drop_graphs(*this);
+  if (cap_st) (void)hipStreamDestroy(cap_st);
+}
+
+// Capture the launches of one cycle on the capture stream and instantiate.
+// Captures are serialised process-wide (contexts on other threads may be
+// capturing too).  Returns false, with no error left pending, when the
+// runtime rejects or invalidates the capture; the caller then launches the
+// cycle directly.
+static std::mutex g_capture_mutex;
+static bool mg_capture(MultiGrid& mg, const double* r, double* z, int ncols, const int* skip,
+                       hipGraphExec_t* exec) {
+  std::lock_guard<std::mutex> lock(g_capture_mutex);
+  *exec = nullptr;
+  if (!mg.cap_st && hipStreamCreateWithFlags(&mg.cap_st, hipStreamNonBlocking) != hipSuccess) {
+    mg.cap_st = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  if (hipStreamBeginCapture(mg.cap_st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const int rc = mg_apply_launches(mg, r, z, ncols, mg.cap_st, skip);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(mg.cap_st, &graph);
+  bool ok = rc == OK && ec == hipSuccess && graph != nullptr;
+  if (ok) ok = hipGraphInstantiateWithFlags(exec, graph, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    *exec = nullptr;
+    (void)hipGetLastError();
+  }
+  return ok;
+}
+
+int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st, const int* skip) {
+  VOX8_CHECK(mg.updated && ncols <= mg.ncols, "multigrid not set up for this density / width");
+  if (!mg.use_graphs) return mg_apply_launches(mg, r, z, ncols, st, skip);
+  MGGraph* hit = nullptr;
+  for (auto& gr : mg.graphs)
+    if (gr.r == r && gr.z == z && gr.ncols == ncols && gr.skip == skip && gr.epoch == mg.epoch)
+      hit = &gr;
+  if (!hit) {
+    // drop stale entries; keep at most 8 live ones (least recently used out)
+    for (size_t i = 0; i < mg.graphs.size();) {
+      if (mg.graphs[i].epoch != mg.epoch) {
+        (void)hipGraphExecDestroy(mg.graphs[i].exec);
+        mg.graphs.erase(mg.graphs.begin() + i);
+      } else {
+        ++i;
+      }
+    }
+    if (mg.graphs.size() >= 8) {
+      size_t old = 0;
+      for (size_t i = 1; i < mg.graphs.size(); ++i)
+        if (mg.graphs[i].last_use < mg.graphs[old].last_use) old = i;
+      (void)hipGraphExecDestroy(mg.graphs[old].exec);
+      mg.graphs.erase(mg.graphs.begin() + old);
+    }
+    MGGraph gr;
+    gr.r = r; gr.z = z; gr.ncols = ncols; gr.skip = skip; gr.epoch = mg.epoch;
+    if (!mg_capture(mg, r, z, ncols, skip, &gr.exec)) {
+      // after repeated rejections stop trying; either way run this cycle directly
+      if (++mg.capture_failures >= 4) mg.use_graphs = false;
+      return mg_apply_launches(mg, r, z, ncols, st, skip);
+    }
+    mg.capture_failures = 0;
+    mg.graphs.push_back(gr);
+    hit = &mg.graphs.back();
+  }
+  hit->last_use = ++mg.use_clock;
+  VOX8_HIP(hipGraphLaunch(hit->exec, st));
   return OK;
 }
 

@@ -584,30 +584,31 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   }
 }
 
-static int zchunk_override() {
-  static const int v = [] {
-    const char* e = getenv("VOX8_ZCHUNK");
-    return e ? atoi(e) : 0;
-  }();
-  return v;
+static int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return e ? atoi(e) : dflt;
 }
 
 static int tiles_x(const Grid& g) { return (g.NX + OX - 1) / OX; }
 static int tiles_y(const Grid& g) { return (g.NY + OY - 1) / OY; }
 
+// Node planes per workgroup: halve the z range until the launch has 384
+// workgroups per column.  Grids with at least that many (tile, plane) pairs
+// keep chunks of >= 4 planes (halo work stays modest; the fine grid gets
+// ~7-plane chunks); smaller grids, the coarse multigrid levels, whose kernels
+// are bound by the latency of the z march, go down to single planes.
+// VOX8_ZCHUNK fixes the chunk.
 int fast_zchunk(const Grid& g) {
-  // z planes per workgroup: halve until the launch has enough workgroups to fill
-  // 256 CUs (at least 4 planes, so the z halo stays a modest overhead)
-  const int ov = zchunk_override();
+  static const int ov = env_int("VOX8_ZCHUNK", 0);
   if (ov > 0) return ov < g.NZ ? ov : g.NZ;
   const int64_t tiles = (int64_t)tiles_x(g) * tiles_y(g);
+  const int zmin = tiles * g.NZ >= 384 ? 4 : 1;
   int z = g.NZ;
-  while (z > 4 && tiles * ((g.NZ + z - 1) / z) < 384) z = (z + 1) / 2;
+  while (z > zmin && tiles * ((g.NZ + z - 1) / z) < 384) z = (z + 1) / 2;
   return z;
 }
 
-int fast_blocks(const Grid& g) {
-  const int zc = fast_zchunk(g);
+int fast_blocks(const Grid& g, int zc) {
   return tiles_x(g) * tiles_y(g) * ((g.NZ + zc - 1) / zc);
 }
 
@@ -620,16 +621,17 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     set_error("structured stencil: grid too large for 32-bit byte offsets (use the dense kernel)");
     return ERR_ARG;
   }
-  const int zc = fast_zchunk(g);
-  const dim3 grid(fast_blocks(g) * nrhs), blk(NT);
+  const dim3 blk(NT);
   const bool fz = fuse.scal != nullptr;
   if (ep.mode != EPI_PLAIN && (op != OP_K || fz)) {
     set_error("stencil epilogue modes apply to K alone");
     return ERR_ARG;
   }
+  const int zc = fast_zchunk(g);
+  const int nb = fast_blocks(g, zc);   // workgroups per column
 #define VOX8_LAUNCH(PAT, K, M, F, MD)                                                            \
-  hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD>), grid, blk, 0, st, g, zc, tiles_x(g),   \
-                     tiles_y(g), nrhs, C, fK, fM, fixed, x, ldx, yK, yM, ldy, fuse, ep)
+  hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD>), dim3(nb * nrhs), blk, 0, st, g, zc,    \
+                     tiles_x(g), tiles_y(g), nrhs, C, fK, fM, fixed, x, ldx, yK, yM, ldy, fuse, ep)
 #define VOX8_PAT(PAT)                                                                   \
   if (op == OP_K) {                                                                     \
     if (ep.mode == EPI_RESID) VOX8_LAUNCH(PAT, true, false, false, EPI_RESID);          \
@@ -645,7 +647,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
 #undef VOX8_PAT
 #undef VOX8_LAUNCH
   VOX8_HIP(hipGetLastError());
-  if (fz) VOX8_TRY(launch_pq_finish(fuse, fast_blocks(g), nrhs, st));
+  if (fz) VOX8_TRY(launch_pq_finish(fuse, nb, nrhs, st));
   return OK;
 }
 
