@@ -59,8 +59,21 @@ struct Epilogue {
   const int* skip = nullptr;  // per column: non-zero = leave this column untouched
 };
 
-int fast_zchunk(const Grid& g);
-int fast_blocks(const Grid& g, int zc);      // workgroups per column
+
+// Element-tile rows (8 or 12) and z chunk (node planes per workgroup).
+struct StencilShape {
+  int ty = 8, zc = 1;
+};
+
+// Autotune the FP32 launch shape for a grid and column count: times the K
+// product of candidate shapes on x (ncols columns, leading dimension ld),
+// writing to scratch (same size); the choice is kept for the process and
+// used by every FP32 launch on such a grid.  Synchronises the stream; not for
+// use during graph capture.
+int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
+                      const uint8_t* fixed, const float* x, int64_t ld, float* scratch, int ncols,
+                      hipStream_t st);
+bool stencil_tuned_fp32(const Grid& g, int ncols);
 
 template <typename T>
 int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,

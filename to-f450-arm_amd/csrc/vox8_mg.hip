@@ -18,6 +18,7 @@
 // Smoothing and transfers run in FP32 on the structured stencil with fused
 // Chebyshev epilogues; the outer Krylov method stays FP64.
 #include <cmath>
+#include <cstdlib>
 
 #include <mutex>
 
@@ -653,6 +654,21 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
         if (mg.graphs[i].last_use < mg.graphs[old].last_use) old = i;
       (void)hipGraphExecDestroy(mg.graphs[old].exec);
       mg.graphs.erase(mg.graphs.begin() + old);
+    }
+    // FP32 stencil shapes of every smoothed level for this width, measured
+    // before the cycle is captured (tuning synchronises; VOX8_TUNE=0 keeps
+    // the fixed rule)
+    static const bool tune = [] {
+      const char* e = getenv("VOX8_TUNE");
+      return !(e && e[0] == '0');
+    }();
+    if (tune) {
+      const size_t nsm = mg.n_coarse ? mg.lv.size() - 1 : mg.lv.size();
+      for (size_t l = 0; l < nsm; ++l) {
+        MGLevel& L = mg.lv[l];
+        VOX8_TRY(stencil_tune_fp32(L.g, L.pattern, L.c32, L.fK, L.fixed, L.x, L.g.n_dofs, L.d2, ncols,
+                                   st));
+      }
     }
     MGGraph gr;
     gr.r = r; gr.z = z; gr.ncols = ncols; gr.skip = skip; gr.epoch = mg.epoch;

@@ -27,15 +27,17 @@
 // Tiles are numbered so that each XCD (workgroups are dealt round-robin to the
 // 8 XCDs) receives a contiguous range, keeping halo re-reads in its L2.
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "vox8_fast.h"
 
 namespace vox8 {
 
-constexpr int TX = 64, TY = 8, NT = TX * TY;            // element tile = threads
-constexpr int OX = TX - 1, OY = TY - 1;                 // owned nodes per tile
-constexpr int PX = TX + 1, PY = TY + 1, PN = PX * PY;   // node plane incl. halo
-constexpr int NLD = (PN + NT - 1) / NT;                 // plane nodes loaded per thread
+// Element tile TX x TY = threads (TY = 8 or 12 element rows, one wavefront
+// each); owned nodes (TX - 1) x (TY - 1); staged node plane (TX + 1) x (TY + 1).
+constexpr int TX = 64, OX = TX - 1, PX = TX + 1;
 constexpr int NXCD = 8;
 
 template <typename T, int PAT>
@@ -285,13 +287,15 @@ __device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mo
 //   xs += d, r -= A d, d' = c1 d + c2 D^-1 r   (all node-local once A d is known);
 // EPI_CHEB0 the same step from xs = 0, r = b with x holding b: d = c0 D^-1 b
 //   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r.
-template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !DO_M ? 6 : 1)))
+template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE, int TY>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !DO_M ? 6 : 1)))
 k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T> C,
              const T* __restrict__ fK,
              const T* __restrict__ fM, const uint8_t* __restrict__ fixed,
              const T* __restrict__ x, int64_t ldx, T* __restrict__ yK, T* __restrict__ yM,
              int64_t ldy, DotFuse fuse, Epilogue<T> ep) {
+  constexpr int NT = TX * TY, OY = TY - 1, PY = TY + 1, PN = PX * PY;
+  constexpr int NLD = (PN + NT - 1) / NT;                  // plane nodes loaded per thread
   constexpr int NOP = (DO_K ? 1 : 0) + (DO_M ? 1 : 0);
   // FP32: three plane slots and two y-exchange buffers, one barrier per layer.
   // FP64 (twice the LDS per value): two slots, one buffer, two barriers.
@@ -590,7 +594,16 @@ static int env_int(const char* name, int dflt) {
 }
 
 static int tiles_x(const Grid& g) { return (g.NX + OX - 1) / OX; }
-static int tiles_y(const Grid& g) { return (g.NY + OY - 1) / OY; }
+static int tiles_y(const Grid& g, int ty) { return (g.NY + ty - 2) / (ty - 1); }
+
+// Rows of the element tile: 12 where that needs fewer element rows over the
+// grid's y extent (less halo work), else 8.  FP32 kernels only: the FP64 ones
+// need more registers than two 12-wavefront workgroups per CU allow.
+template <typename T>
+static int tile_rows(const Grid& g) {
+  if (sizeof(T) != 4) return 8;
+  return tiles_y(g, 12) * 12 < tiles_y(g, 8) * 8 ? 12 : 8;
+}
 
 // Node planes per workgroup: halve the z range until the launch has 384
 // workgroups per column.  Grids with at least that many (tile, plane) pairs
@@ -598,18 +611,137 @@ static int tiles_y(const Grid& g) { return (g.NY + OY - 1) / OY; }
 // ~7-plane chunks); smaller grids, the coarse multigrid levels, whose kernels
 // are bound by the latency of the z march, go down to single planes.
 // VOX8_ZCHUNK fixes the chunk.
-int fast_zchunk(const Grid& g) {
+static int fast_zchunk(const Grid& g, int ty) {
   static const int ov = env_int("VOX8_ZCHUNK", 0);
   if (ov > 0) return ov < g.NZ ? ov : g.NZ;
-  const int64_t tiles = (int64_t)tiles_x(g) * tiles_y(g);
+  const int64_t tiles = (int64_t)tiles_x(g) * tiles_y(g, ty);
   const int zmin = tiles * g.NZ >= 384 ? 4 : 1;
   int z = g.NZ;
   while (z > zmin && tiles * ((g.NZ + z - 1) / z) < 384) z = (z + 1) / 2;
   return z;
 }
 
-int fast_blocks(const Grid& g, int zc) {
-  return tiles_x(g) * tiles_y(g) * ((g.NZ + zc - 1) / zc);
+static int fast_blocks(const Grid& g, int ty, int zc) {
+  return tiles_x(g) * tiles_y(g, ty) * ((g.NZ + zc - 1) / zc);
+}
+
+// ---- shape autotuner (FP32) ----------------------------------------------
+// The (tile rows, z chunk) of an FP32 launch is measured rather than guessed:
+// for a grid and column count, stencil_tune_fp32 times the K product of each
+// candidate shape on the caller's data (output to a scratch buffer) and keeps
+// the fastest.  An FP32 result does not depend on the shape (every node's sum
+// is formed in the same order whatever the tiling), so tuning changes no
+// value.  FP64 launches (PCG and LOBPCG products, whose fused dot products
+// are summed per workgroup) keep the fixed rule above, which keeps those
+// results independent of timing.
+struct ShapeKey {
+  int nx, ny, nz, ncols;
+  bool operator<(const ShapeKey& o) const {
+    if (nx != o.nx) return nx < o.nx;
+    if (ny != o.ny) return ny < o.ny;
+    if (nz != o.nz) return nz < o.nz;
+    return ncols < o.ncols;
+  }
+};
+static std::mutex g_shape_mutex;
+static std::map<ShapeKey, StencilShape> g_shapes;
+
+template <typename T>
+static StencilShape stencil_shape(const Grid& g, int ncols) {
+  if (sizeof(T) == 4) {
+    std::lock_guard<std::mutex> lock(g_shape_mutex);
+    const auto it = g_shapes.find(ShapeKey{g.nx, g.ny, g.nz, ncols});
+    if (it != g_shapes.end()) return it->second;
+  }
+  StencilShape sh;
+  sh.ty = tile_rows<T>(g);
+  sh.zc = fast_zchunk(g, sh.ty);
+  return sh;
+}
+
+bool stencil_tuned_fp32(const Grid& g, int ncols) {
+  std::lock_guard<std::mutex> lock(g_shape_mutex);
+  return g_shapes.count(ShapeKey{g.nx, g.ny, g.nz, ncols}) != 0;
+}
+
+template <int TY>
+static int time_shape(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
+                      const uint8_t* fixed, const float* x, int64_t ld, float* y, int ncols, int zc,
+                      hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
+  const int nb = fast_blocks(g, TY, zc);
+  auto launch = [&]() {
+    if (pattern == PAT_CUBE)
+      hipLaunchKernelGGL((k_apply_fast<float, PAT_CUBE, true, false, false, EPI_PLAIN, TY>),
+                         dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),
+                         ncols, C, fK, (const float*)nullptr, fixed, x, ld, y, (float*)nullptr, ld,
+                         DotFuse(), Epilogue<float>());
+    else if (pattern == PAT_ORTHO)
+      hipLaunchKernelGGL((k_apply_fast<float, PAT_ORTHO, true, false, false, EPI_PLAIN, TY>),
+                         dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),
+                         ncols, C, fK, (const float*)nullptr, fixed, x, ld, y, (float*)nullptr, ld,
+                         DotFuse(), Epilogue<float>());
+    else
+      hipLaunchKernelGGL((k_apply_fast<float, PAT_BLOCK, true, false, false, EPI_PLAIN, TY>),
+                         dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),
+                         ncols, C, fK, (const float*)nullptr, fixed, x, ld, y, (float*)nullptr, ld,
+                         DotFuse(), Epilogue<float>());
+  };
+  launch();                                               // warm-up
+  VOX8_HIP(hipEventRecord(e0, st));
+  constexpr int REPS = 3;
+  for (int i = 0; i < REPS; ++i) launch();
+  VOX8_HIP(hipEventRecord(e1, st));
+  VOX8_HIP(hipGetLastError());
+  VOX8_HIP(hipEventSynchronize(e1));
+  VOX8_HIP(hipEventElapsedTime(ms, e0, e1));
+  *ms /= REPS;
+  return OK;
+}
+
+int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
+                      const uint8_t* fixed, const float* x, int64_t ld, float* scratch, int ncols,
+                      hipStream_t st) {
+  if (stencil_tuned_fp32(g, ncols)) return OK;
+  VOX8_CHECK(pattern != PAT_DENSE, "shape tuning needs the structured stencil");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VOX8_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("hipEventCreate failed");
+    return ERR_HIP;
+  }
+  StencilShape best{tile_rows<float>(g), fast_zchunk(g, tile_rows<float>(g))};
+  float best_ms = 1e30f;
+  int rc = OK;
+  for (int ty : {8, 12}) {
+    // distinct z chunkings: the rule's choice and up to 6 others around it
+    std::vector<int> zcs;
+    for (int chunks = 1; chunks <= g.NZ && (int)zcs.size() < 12; ++chunks) {
+      const int z = (g.NZ + chunks - 1) / chunks;
+      if ((g.NZ + z - 1) / z != chunks) continue;
+      const int64_t wgs = (int64_t)tiles_x(g) * tiles_y(g, ty) * chunks * ncols;
+      if (wgs < 64 && z > 1) continue;            // far too few workgroups
+      if (wgs > 16384) break;                     // far too many
+      zcs.push_back(z);
+    }
+    for (int z : zcs) {
+      float ms = 0.f;
+      rc = ty == 12 ? time_shape<12>(g, pattern, C, fK, fixed, x, ld, scratch, ncols, z, st, e0, e1, &ms)
+                    : time_shape<8>(g, pattern, C, fK, fixed, x, ld, scratch, ncols, z, st, e0, e1, &ms);
+      if (rc != OK) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = StencilShape{ty, z};
+      }
+    }
+    if (rc != OK) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != OK) return rc;
+  std::lock_guard<std::mutex> lock(g_shape_mutex);
+  g_shapes[ShapeKey{g.nx, g.ny, g.nz, ncols}] = best;
+  return OK;
 }
 
 template <typename T>
@@ -621,17 +753,28 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     set_error("structured stencil: grid too large for 32-bit byte offsets (use the dense kernel)");
     return ERR_ARG;
   }
-  const dim3 blk(NT);
   const bool fz = fuse.scal != nullptr;
   if (ep.mode != EPI_PLAIN && (op != OP_K || fz)) {
     set_error("stencil epilogue modes apply to K alone");
     return ERR_ARG;
   }
-  const int zc = fast_zchunk(g);
-  const int nb = fast_blocks(g, zc);   // workgroups per column
+  const StencilShape shp = stencil_shape<T>(g, nrhs);
+  const int ty = shp.ty, zc = shp.zc;
+  const int nb = fast_blocks(g, ty, zc);   // workgroups per column
+#define VOX8_LAUNCH_TY(PAT, K, M, F, MD, TYV)                                                     \
+  hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD, TYV>), dim3(nb * nrhs), dim3(TX * TYV), 0, \
+                     st, g, zc, tiles_x(g), tiles_y(g, TYV), nrhs, C, fK, fM, fixed, x, ldx, yK, yM,\
+                     ldy, fuse, ep)
 #define VOX8_LAUNCH(PAT, K, M, F, MD)                                                            \
-  hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD>), dim3(nb * nrhs), blk, 0, st, g, zc,    \
-                     tiles_x(g), tiles_y(g), nrhs, C, fK, fM, fixed, x, ldx, yK, yM, ldy, fuse, ep)
+  do {                                                                                           \
+    if constexpr (sizeof(T) == 4) {                                                              \
+      if (ty == 12) {                                                                            \
+        VOX8_LAUNCH_TY(PAT, K, M, F, MD, 12);                                                    \
+        break;                                                                                   \
+      }                                                                                          \
+    }                                                                                            \
+    VOX8_LAUNCH_TY(PAT, K, M, F, MD, 8);                                                         \
+  } while (0)
 #define VOX8_PAT(PAT)                                                                   \
   if (op == OP_K) {                                                                     \
     if (ep.mode == EPI_RESID) VOX8_LAUNCH(PAT, true, false, false, EPI_RESID);          \
@@ -646,6 +789,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
   else { VOX8_PAT(PAT_BLOCK) }
 #undef VOX8_PAT
 #undef VOX8_LAUNCH
+#undef VOX8_LAUNCH_TY
   VOX8_HIP(hipGetLastError());
   if (fz) VOX8_TRY(launch_pq_finish(fuse, nb, nrhs, st));
   return OK;
