@@ -65,14 +65,14 @@ struct StencilShape {
   int ty = 8, zc = 1;
 };
 
-// Autotune the FP32 launch shape for a grid and column count: times the K
-// product of candidate shapes on x (ncols columns, leading dimension ld),
-// writing to scratch (same size); the choice is kept for the process and
-// used by every FP32 launch on such a grid.  Synchronises the stream; not for
-// use during graph capture.
+// Autotune the FP32 launch shape for a grid and column count: times a
+// Chebyshev smoothing step of each candidate shape on d, r, xs, dnew (ncols
+// columns, leading dimension ld; r, xs, dnew are overwritten, so pass work
+// vectors) and keeps the choice for the process: every FP32 launch on such a
+// grid uses it.  Synchronises the stream; not for use during graph capture.
 int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
-                      const uint8_t* fixed, const float* x, int64_t ld, float* scratch, int ncols,
-                      hipStream_t st);
+                      const uint8_t* fixed, const float* dinv, float* d, float* r, float* xs,
+                      float* dnew, int64_t ld, int ncols, hipStream_t st);
 bool stencil_tuned_fp32(const Grid& g, int ncols);
 
 template <typename T>

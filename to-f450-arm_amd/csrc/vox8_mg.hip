@@ -656,8 +656,9 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
       mg.graphs.erase(mg.graphs.begin() + old);
     }
     // FP32 stencil shapes of every smoothed level for this width, measured
-    // before the cycle is captured (tuning synchronises; VOX8_TUNE=0 keeps
-    // the fixed rule)
+    // before the cycle is captured on the level's work vectors (their
+    // contents are dead between cycles; tuning synchronises; VOX8_TUNE=0
+    // keeps the fixed rule)
     static const bool tune = [] {
       const char* e = getenv("VOX8_TUNE");
       return !(e && e[0] == '0');
@@ -666,8 +667,8 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
       const size_t nsm = mg.n_coarse ? mg.lv.size() - 1 : mg.lv.size();
       for (size_t l = 0; l < nsm; ++l) {
         MGLevel& L = mg.lv[l];
-        VOX8_TRY(stencil_tune_fp32(L.g, L.pattern, L.c32, L.fK, L.fixed, L.x, L.g.n_dofs, L.d2, ncols,
-                                   st));
+        VOX8_TRY(stencil_tune_fp32(L.g, L.pattern, L.c32, L.fK, L.fixed, L.dinv, L.d, L.r, L.x,
+                                   L.d2, L.g.n_dofs, ncols, st));
       }
     }
     MGGraph gr;

@@ -26,6 +26,8 @@
 // once; the summation order is fixed, so results are bitwise reproducible.
 // Tiles are numbered so that each XCD (workgroups are dealt round-robin to the
 // 8 XCDs) receives a contiguous range, keeping halo re-reads in its L2.
+#include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -627,9 +629,9 @@ static int fast_blocks(const Grid& g, int ty, int zc) {
 
 // ---- shape autotuner (FP32) ----------------------------------------------
 // The (tile rows, z chunk) of an FP32 launch is measured rather than guessed:
-// for a grid and column count, stencil_tune_fp32 times the K product of each
-// candidate shape on the caller's data (output to a scratch buffer) and keeps
-// the fastest.  An FP32 result does not depend on the shape (every node's sum
+// for a grid and column count, stencil_tune_fp32 times a Chebyshev smoothing
+// step (the multigrid's kernel) of each candidate shape on the caller's work
+// vectors and keeps the fastest.  An FP32 result does not depend on the shape (every node's sum
 // is formed in the same order whatever the tiling), so tuning changes no
 // value.  FP64 launches (PCG and LOBPCG products, whose fused dot products
 // are summed per workgroup) keep the fixed rule above, which keeps those
@@ -666,26 +668,21 @@ bool stencil_tuned_fp32(const Grid& g, int ncols) {
 
 template <int TY>
 static int time_shape(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
-                      const uint8_t* fixed, const float* x, int64_t ld, float* y, int ncols, int zc,
-                      hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
+                      const uint8_t* fixed, const float* d, const Epilogue<float>& ep, int ncols,
+                      int zc, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms) {
   const int nb = fast_blocks(g, TY, zc);
+  const int64_t ld = ep.ld;
+#define VOX8_TUNE_LAUNCH(PAT)                                                                   \
+  hipLaunchKernelGGL((k_apply_fast<float, PAT, true, false, false, EPI_CHEB, TY>),               \
+                     dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),  \
+                     ncols, C, fK, (const float*)nullptr, fixed, d, ld, (float*)nullptr,         \
+                     (float*)nullptr, ld, DotFuse(), ep)
   auto launch = [&]() {
-    if (pattern == PAT_CUBE)
-      hipLaunchKernelGGL((k_apply_fast<float, PAT_CUBE, true, false, false, EPI_PLAIN, TY>),
-                         dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),
-                         ncols, C, fK, (const float*)nullptr, fixed, x, ld, y, (float*)nullptr, ld,
-                         DotFuse(), Epilogue<float>());
-    else if (pattern == PAT_ORTHO)
-      hipLaunchKernelGGL((k_apply_fast<float, PAT_ORTHO, true, false, false, EPI_PLAIN, TY>),
-                         dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),
-                         ncols, C, fK, (const float*)nullptr, fixed, x, ld, y, (float*)nullptr, ld,
-                         DotFuse(), Epilogue<float>());
-    else
-      hipLaunchKernelGGL((k_apply_fast<float, PAT_BLOCK, true, false, false, EPI_PLAIN, TY>),
-                         dim3(nb * ncols), dim3(TX * TY), 0, st, g, zc, tiles_x(g), tiles_y(g, TY),
-                         ncols, C, fK, (const float*)nullptr, fixed, x, ld, y, (float*)nullptr, ld,
-                         DotFuse(), Epilogue<float>());
+    if (pattern == PAT_CUBE) VOX8_TUNE_LAUNCH(PAT_CUBE);
+    else if (pattern == PAT_ORTHO) VOX8_TUNE_LAUNCH(PAT_ORTHO);
+    else VOX8_TUNE_LAUNCH(PAT_BLOCK);
   };
+#undef VOX8_TUNE_LAUNCH
   launch();                                               // warm-up
   VOX8_HIP(hipEventRecord(e0, st));
   constexpr int REPS = 3;
@@ -699,9 +696,18 @@ static int time_shape(const Grid& g, int pattern, const FastCoef<float>& C, cons
 }
 
 int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
-                      const uint8_t* fixed, const float* x, int64_t ld, float* scratch, int ncols,
-                      hipStream_t st) {
+                      const uint8_t* fixed, const float* dinv, float* d, float* r, float* xs,
+                      float* dnew, int64_t ld, int ncols, hipStream_t st) {
   if (stencil_tuned_fp32(g, ncols)) return OK;
+  Epilogue<float> ep;
+  ep.mode = EPI_CHEB;
+  ep.r = r;
+  ep.xs = xs;
+  ep.dnew = dnew;
+  ep.dinv = dinv;
+  ep.c1 = 0.5f;
+  ep.c2 = 1e-3f;
+  ep.ld = ld;
   VOX8_CHECK(pattern != PAT_DENSE, "shape tuning needs the structured stencil");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   VOX8_HIP(hipEventCreate(&e0));
@@ -710,8 +716,11 @@ int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, cons
     set_error("hipEventCreate failed");
     return ERR_HIP;
   }
-  StencilShape best{tile_rows<float>(g), fast_zchunk(g, tile_rows<float>(g))};
-  float best_ms = 1e30f;
+  struct Cand {
+    int ty, zc;
+    float ms;
+  };
+  std::vector<Cand> done;
   int rc = OK;
   for (int ty : {8, 12}) {
     // distinct z chunkings: the rule's choice and up to 6 others around it
@@ -726,19 +735,38 @@ int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, cons
     }
     for (int z : zcs) {
       float ms = 0.f;
-      rc = ty == 12 ? time_shape<12>(g, pattern, C, fK, fixed, x, ld, scratch, ncols, z, st, e0, e1, &ms)
-                    : time_shape<8>(g, pattern, C, fK, fixed, x, ld, scratch, ncols, z, st, e0, e1, &ms);
+      rc = ty == 12 ? time_shape<12>(g, pattern, C, fK, fixed, d, ep, ncols, z, st, e0, e1, &ms)
+                    : time_shape<8>(g, pattern, C, fK, fixed, d, ep, ncols, z, st, e0, e1, &ms);
       if (rc != OK) break;
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = StencilShape{ty, z};
-      }
+      done.push_back(Cand{ty, z, ms});
     }
     if (rc != OK) break;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != OK) return rc;
+  // Candidates within 3 % of the fastest count as equal (run-to-run timing
+  // noise); among those the one computing the fewest elements (least halo
+  // work and traffic) wins, so the choice is stable across runs.
+  StencilShape best{tile_rows<float>(g), fast_zchunk(g, tile_rows<float>(g))};
+  float fastest = 1e30f;
+  for (const Cand& c : done) fastest = std::min(fastest, c.ms);
+  int64_t best_work = INT64_MAX;
+  for (const Cand& c : done) {
+    if (c.ms > 1.03f * fastest) continue;
+    const int64_t work = (int64_t)tiles_x(g) * tiles_y(g, c.ty) * c.ty * ((g.NZ + c.zc - 1) / c.zc) *
+                         (c.zc + 1);
+    if (work < best_work || (work == best_work && c.zc > best.zc)) {
+      best_work = work;
+      best = StencilShape{c.ty, c.zc};
+    }
+  }
+  static const bool log = env_int("VOX8_TUNE_LOG", 0) != 0;
+  if (log) {
+    fprintf(stderr, "[vox8 tune] %dx%dx%d x%d:", g.nx, g.ny, g.nz, ncols);
+    for (const Cand& c : done) fprintf(stderr, " %d/%d:%.1fus", c.ty, c.zc, 1e3f * c.ms);
+    fprintf(stderr, " -> %d/%d\n", best.ty, best.zc);
+  }
   std::lock_guard<std::mutex> lock(g_shape_mutex);
   g_shapes[ShapeKey{g.nx, g.ny, g.nz, ncols}] = best;
   return OK;
