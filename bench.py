@@ -127,13 +127,21 @@ def kernel_rooflines(fem, geom, rho, nrhs=6, reps=30):
     alg = 16 * n + 8 * ne + nn
     out["k_fp64"] = dict(kernel="k_apply_fast<double,CUBE,K> (FP64 K product, 1 column)", t=t, alg=alg,
                          per_unit="16 B per DOF (read x; write Kx), + 8 B/element factor, 1 B/node mask")
+    # attainable streaming rate on this part: a 1 GiB device-to-device copy
+    # (read + write bytes), timed the same way
+    src = torch.ones(1 << 27, dtype=torch.float64, device=fem.device)
+    dst = torch.empty_like(src)
+    tcopy = _event_time(fem, lambda: dst.copy_(src), 5)
+    copy_gbs = 2 * src.numel() * 8 / tcopy / 1e9
+    del src, dst
     res = {}
     for k, v in out.items():
         ach = v["alg"] / v["t"] / 1e9
         res[k] = {"kernel": v["kernel"], "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                   "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic(k, geom),
                   "avg_launch_us": round(v["t"] * 1e6, 2), "alg_bytes_per_launch": int(v["alg"]),
-                  "alg_bytes_per_unit": v["per_unit"]}
+                  "alg_bytes_per_unit": v["per_unit"], "copy_GBs": round(copy_gbs, 1),
+                  "frac_of_copy": round(ach / copy_gbs, 4)}
     return res
 
 
