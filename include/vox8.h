@@ -126,6 +126,22 @@ int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, 
  * nu 2, alpha 12, coarse_dofs 700, cycle 2, w_min_dofs 50000. */
 int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, int cycle,
                         int64_t w_min_dofs);
+/* Centre-point element stresses, Voigt order (xx, yy, zz, xy, yz, xz), linear
+ * in the density: d_stress[6e + i] = rho[e] (D B u_e)_i for every element of
+ * the grid (0 on inactive elements).  h_B is the 6x24 centre
+ * strain-displacement matrix (row-major, columns in the element's local DOF
+ * order), h_D the 6x6 constitutive matrix, as the reference builds them.
+ * (FEM3D.compute_stress, src/fem.py:564-579) */
+int vox8_stress(vox8_ctx* ctx, const double* d_u, const double* d_rho_grid, const double* h_B,
+                const double* h_D, double* d_stress);
+
+/* Von Mises stress and largest principal stress of n Voigt rows of any device
+ * array (either output may be null), on the given HIP stream (null: the
+ * default stream); no context needed.  (FEM3D.von_mises and
+ * principal_stress_max, src/fem.py:582-602) */
+int vox8_stress_invariants(void* hip_stream, int64_t n, const double* d_stress, double* d_vm,
+                           double* d_pmax);
+
 /* Replay each multigrid cycle as a captured HIP graph (default 1) or launch
  * its kernels one by one (0).  A cycle is captured once per (r, z, nrhs,
  * skip mask) and hierarchy state and replayed afterwards; results are the

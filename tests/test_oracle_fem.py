@@ -242,3 +242,32 @@ def test_effective_mass(which):
         r[fem.fixed_dofs] = 0.0
         eff = effective_mass_voxel(fem, geom, rho, (3.7 * r)[:, None], 1e-4, tip_nodes=tip)
         assert abs(eff[0] - 1) < 1e-13
+
+
+def test_stress_affine_fields_and_invariants_closed_form():
+    """Centre stresses of affine displacement fields (reproduced exactly by
+    trilinear elements) equal rho D eps; von Mises and the largest principal
+    stress of known tensors (uniaxial, pure shear, a rotated diagonal)."""
+    geom = MaskedBox(4, 3, 2, h=2e-3, hole_frac=0.2, seed=9)
+    fem = _fem(geom)
+    E, nu = MAT["E"], MAT["nu"]
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    idx = np.arange(geom.n_nodes)
+    X = np.stack([(idx % (geom.nx + 1)) * geom.dx, ((idx // (geom.nx + 1)) % (geom.ny + 1)) * geom.dy,
+                  (idx // ((geom.nx + 1) * (geom.ny + 1))) * geom.dz], axis=1)
+    rho = random_density(fem.n_active, 3, 0.2, 1.0)
+    eps0, gam = 1e-4, 3e-4
+    u = np.zeros((geom.n_nodes, 3))
+    u[:, 0] = eps0 * X[:, 0] + gam * X[:, 1]          # eps_xx = eps0, engineering gamma_xy = gam
+    s = fem.compute_stress(u.ravel(), rho)
+    expect = np.array([(lam + 2 * mu) * eps0, lam * eps0, lam * eps0, mu * gam, 0.0, 0.0])
+    assert rel(s, rho[:, None] * expect[None, :]) < 1e-12
+    # invariants of known tensors
+    sig = 7.5e6
+    T = np.array([[sig, 0, 0, 0, 0, 0], [0, 0, 0, sig, 0, 0], [-sig, -sig, -sig, 0, 0, 0]])
+    vm = fem.von_mises(T)
+    assert vm == pytest.approx([sig, np.sqrt(3) * sig, 0.0], abs=1e-6)
+    R = sla.expm(np.array([[0, -0.3, 0.2], [0.3, 0, -0.5], [-0.2, 0.5, 0]]))   # a rotation
+    A = R @ np.diag([1.0, 2.0, 3.0]) @ R.T
+    row = np.array([[A[0, 0], A[1, 1], A[2, 2], A[0, 1], A[1, 2], A[0, 2]]])
+    assert fem.principal_stress_max(row)[0] == pytest.approx(3.0, rel=1e-13)

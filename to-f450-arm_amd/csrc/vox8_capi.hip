@@ -9,6 +9,7 @@
 #include "vox8_fast.h"
 #include "vox8_kernels.h"
 #include "vox8_mg.h"
+#include "vox8_post.h"
 #include "vox8_pcg.h"
 #include "vox8_topopt.h"
 
@@ -652,6 +653,26 @@ int vox8_filter_setup(vox8_ctx* c, double r_min_vox, const uint8_t* d_mask) {
   VOX8_HIP(hipStreamSynchronize(c->st));
   c->have_filter = true;
   return OK;
+  API_END
+}
+
+int vox8_stress(vox8_ctx* c, const double* d_u, const double* d_rho_grid, const double* h_B,
+                const double* h_D, double* d_stress) {
+  API_BEGIN
+  VOX8_TRY(require(c, false));
+  VOX8_CHECK(d_u && d_rho_grid && h_B && h_D && d_stress, "null argument");
+  StressCoef sc;
+  for (int i = 0; i < 6 * 24; ++i) sc.B[i] = h_B[i];
+  for (int i = 0; i < 36; ++i) sc.D[i] = h_D[i];
+  return launch_stress(c->g, sc, d_u, d_rho_grid, c->d_active, d_stress, c->st);
+  API_END
+}
+
+int vox8_stress_invariants(void* hip_stream, int64_t n, const double* d_stress, double* d_vm,
+                           double* d_pmax) {
+  API_BEGIN
+  VOX8_CHECK(n >= 0 && (n == 0 || d_stress), "bad argument");
+  return launch_stress_invariants(n, d_stress, d_vm, d_pmax, (hipStream_t)hip_stream);
   API_END
 }
 

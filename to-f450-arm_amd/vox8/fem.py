@@ -59,6 +59,15 @@ def _tip_nodes_of(geom):
     return np.array([], dtype=np.int64)
 
 
+def _stress_to_dev(stress):
+    """(n, 6) device float64 tensor of Voigt rows, converter back to the caller's type."""
+    if isinstance(stress, torch.Tensor):
+        t = stress if stress.is_cuda else stress.to("cuda")
+        return t.to(torch.float64).reshape(-1, 6).contiguous(), (lambda y: y)
+    arr = np.ascontiguousarray(np.asarray(stress, dtype=np.float64).reshape(-1, 6))
+    return torch.from_numpy(arr).to("cuda"), (lambda y: y.cpu().numpy())
+
+
 class VoxelOperator:
     """Matrix-free K(rho) or M(rho) on the full DOF vector (what assemble_* return).
 
@@ -429,28 +438,38 @@ class FEM3D:
 
     # ------------------------------------------------------------------ post-processing
     def compute_stress(self, u, rho_active):
-        """Centre-point Voigt stress, linear in rho (src/fem.py:564-579)."""
-        u = np.asarray(u.cpu().numpy() if isinstance(u, torch.Tensor) else u, dtype=float)
-        rho = np.asarray(rho_active.cpu().numpy() if isinstance(rho_active, torch.Tensor) else rho_active)
-        ue = u[self.elem_dof_ids]
-        return rho[:, None] * ((ue @ self.B_centre.T) @ self.D_solid.T)
+        """Centre-point Voigt stress per active element, linear in rho
+        (src/fem.py:564-579): rho_e D (B u_e), shape (n_active, 6), on the
+        device (vox8_stress); numpy in, numpy out."""
+        ud, back = self._to_dev(u)
+        rg = self.grid_from_active(rho_active)
+        out = torch.empty((self.geom.n_elements, 6), dtype=torch.float64, device=self.device)
+        B = np.ascontiguousarray(self.B_centre, dtype=np.float64)
+        D = np.ascontiguousarray(self.D_solid, dtype=np.float64)
+        self._call("vox8_stress", ptr(ud.reshape(-1)), ptr(rg), _lib.f64p(B), _lib.f64p(D), ptr(out))
+        return back(self.active_from_grid(out))
+
+    @staticmethod
+    def _invariants(stress):
+        st, back = _stress_to_dev(stress)
+        n = st.shape[0]
+        vm = torch.empty(n, dtype=torch.float64, device=st.device)
+        pm = torch.empty(n, dtype=torch.float64, device=st.device)
+        call("vox8_stress_invariants", C.c_void_p(torch.cuda.current_stream(st.device).cuda_stream), n,
+             ptr(st), ptr(vm), ptr(pm))
+        return back(vm), back(pm)
 
     @staticmethod
     def von_mises(stress):
-        s = stress
-        vm2 = 0.5 * ((s[..., 0] - s[..., 1]) ** 2 + (s[..., 1] - s[..., 2]) ** 2
-                     + (s[..., 2] - s[..., 0]) ** 2) + 3.0 * (s[..., 3] ** 2 + s[..., 4] ** 2 + s[..., 5] ** 2)
-        return np.sqrt(np.maximum(vm2, 0.0))
+        """Von Mises equivalent stress of Voigt rows (src/fem.py:582-590), on
+        the device (vox8_stress_invariants)."""
+        return FEM3D._invariants(stress)[0]
 
     @staticmethod
     def principal_stress_max(stress):
-        s = np.asarray(stress)
-        T = np.empty((s.shape[0], 3, 3))
-        T[:, 0, 0], T[:, 1, 1], T[:, 2, 2] = s[:, 0], s[:, 1], s[:, 2]
-        T[:, 0, 1] = T[:, 1, 0] = s[:, 3]
-        T[:, 1, 2] = T[:, 2, 1] = s[:, 4]
-        T[:, 0, 2] = T[:, 2, 0] = s[:, 5]
-        return np.linalg.eigvalsh(T).max(axis=1)
+        """Largest principal stress of Voigt rows (src/fem.py:593-602): closed
+        form of the symmetric 3x3 eigenvalue problem, on the device."""
+        return FEM3D._invariants(stress)[1]
 
     def total_mass(self, rho_active) -> float:
         V_e = self.geom.dx * self.geom.dy * self.geom.dz
