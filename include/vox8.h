@@ -142,6 +142,18 @@ int vox8_stress(vox8_ctx* ctx, const double* d_u, const double* d_rho_grid, cons
 int vox8_stress_invariants(void* hip_stream, int64_t n, const double* d_stress, double* d_vm,
                            double* d_pmax);
 
+/* Goodman / S-N fatigue fields of two load-case stress fields (n elements,
+ * Pa): mean, alternating, Goodman-equivalent alternating stress, life,
+ * damage 1/max(life, 1) and factor of safety fos_ref_mpa*1e6/eq (outputs may be
+ * null).  The S-N curve is a 7-point table of log10(stress, MPa), strictly
+ * increasing, and log10(cycles); life = 1e9 below 0.99 endurance_mpa and 1
+ * above 1.5 overload_mpa.  No context needed.  (fatigue_life_field and
+ * sn_life, src/fatigue.py:24-121) */
+int vox8_fatigue(void* hip_stream, int64_t n, const double* d_s1, const double* d_s2, double uts_pa,
+                 const double* h_log_stress_mpa, const double* h_log_cycles, double endurance_mpa,
+                 double overload_mpa, double fos_ref_mpa, double* d_mean, double* d_alt, double* d_eq,
+                 double* d_life, double* d_damage, double* d_fos);
+
 /* Replay each multigrid cycle as a captured HIP graph (default 1) or launch
  * its kernels one by one (0).  A cycle is captured once per (r, z, nrhs,
  * skip mask) and hierarchy state and replayed afterwards; results are the

@@ -62,3 +62,30 @@ def test_invariants_special_tensors(gpu):
     vm_ref = np.sqrt(np.maximum(0.5 * ((s[:, 0] - s[:, 1]) ** 2 + (s[:, 1] - s[:, 2]) ** 2
                                        + (s[:, 2] - s[:, 0]) ** 2) + 3 * (s[:, 3:] ** 2).sum(1), 0))
     assert rel(vm, vm_ref) < 1e-13
+
+
+def test_fatigue_known_answers_and_oracle(gpu):
+    """vox8_fatigue: the reference's OEM known answer and hand cases
+    (src/fatigue.py:124-141), and random stress fields against the oracle."""
+    import json
+    from conftest import GOLDEN
+    from oracle import fatigue as of
+    from vox8 import fatigue as vf
+    d = json.loads((GOLDEN / "reference_notes.json").read_text())["oem_baseline"]
+    r = vf.fatigue_life_field([d["max_vm_MPa_LC1"] * 1e6], [d["max_vm_MPa_LC2"] * 1e6], d["sigma_u_MPa"] * 1e6)
+    assert r["factor_of_safety"][0] == pytest.approx(d["sf_fatigue"], rel=1e-14)
+    assert r["life"][0] == d["N_cycles"]
+    for (alt, mean), eq, life in [((15e6, 15e6), 21.818182e6, 5.513165e4), ((6e6, 6e6), 6.857143e6, 1e9),
+                                  ((40e6, 5e6), 44.651163e6, 111.1802)]:
+        e = vf.equivalent_alt_stress_goodman(np.array([alt]), np.array([mean]), 48e6)
+        assert e[0] == pytest.approx(eq, rel=1e-7)
+        assert vf.sn_life(e)[0] == pytest.approx(life, rel=1e-6)
+    rng = np.random.default_rng(3)
+    s1 = np.abs(rng.standard_normal(5000)) * 2e7
+    s2 = s1 * rng.uniform(-1, 1, 5000)
+    s1[:3] = [0.0, 11.8e6, 70e6]          # zero, below endurance, gross overload
+    a, b = vf.fatigue_life_field(s1, s2, 48e6), of.fatigue_life_field(s1, s2, 48e6)
+    for k in b:
+        assert rel(a[k], b[k]) < 1e-12, k
+    t = vf.fatigue_life_field(torch.tensor(s1, device=gpu), torch.tensor(s2, device=gpu), 48e6)
+    assert t["life"].is_cuda and rel(t["life"].cpu().numpy(), b["life"]) < 1e-12

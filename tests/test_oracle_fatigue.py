@@ -1,6 +1,7 @@
-"""Oracle fatigue (oracle/fatigue.py) and product fatigue (vox8/fatigue.py)
-against the reference's committed OEM known answer (notes/oem_baseline.json)
-and the hand-worked cases of src/fatigue.py:124-141."""
+"""Oracle fatigue (oracle/fatigue.py) against the reference's committed OEM
+known answer (notes/oem_baseline.json) and the hand-worked cases of
+src/fatigue.py:124-141 (the device implementation is checked against the same
+cases in tests/test_gpu_post.py)."""
 import json
 
 import numpy as np
@@ -11,8 +12,7 @@ from oracle import fatigue as of
 
 
 def _impls():
-    from vox8 import fatigue as vf
-    return (of, vf)
+    return (of,)
 
 
 @pytest.mark.parametrize("impl", _impls())

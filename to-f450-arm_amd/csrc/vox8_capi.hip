@@ -676,6 +676,21 @@ int vox8_stress_invariants(void* hip_stream, int64_t n, const double* d_stress, 
   API_END
 }
 
+int vox8_fatigue(void* hip_stream, int64_t n, const double* d_s1, const double* d_s2, double uts_pa,
+                 const double* h_log_stress_mpa, const double* h_log_cycles, double endurance_mpa,
+                 double overload_mpa, double fos_ref_mpa, double* d_mean, double* d_alt, double* d_eq,
+                 double* d_life, double* d_damage, double* d_fos) {
+  API_BEGIN
+  VOX8_CHECK(n >= 0 && (n == 0 || (d_s1 && d_s2)) && h_log_stress_mpa && h_log_cycles && uts_pa > 0.0,
+             "bad argument");
+  for (int i = 1; i < 7; ++i)
+    VOX8_CHECK(h_log_stress_mpa[i] > h_log_stress_mpa[i - 1], "S-N stress table must increase");
+  return launch_fatigue(n, d_s1, d_s2, uts_pa, h_log_stress_mpa, h_log_cycles, endurance_mpa,
+                        overload_mpa, fos_ref_mpa, d_mean, d_alt, d_eq, d_life, d_damage, d_fos,
+                        (hipStream_t)hip_stream);
+  API_END
+}
+
 int vox8_filter(vox8_ctx* c, const double* d_in, double* d_out) {
   API_BEGIN
   VOX8_CHECK(c && d_in && d_out, "null argument");

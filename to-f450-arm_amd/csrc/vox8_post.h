@@ -14,5 +14,10 @@ struct StressCoef {
 int launch_stress(const Grid& g, const StressCoef& sc, const double* u, const double* rho,
                   const uint8_t* active, double* out, hipStream_t st);
 int launch_stress_invariants(int64_t n, const double* s, double* vm, double* pmax, hipStream_t st);
+// Goodman / S-N fields; h_xs / h_ys the 7-point log10 table (stress ascending).
+int launch_fatigue(int64_t n, const double* s1, const double* s2, double uts, const double* h_xs,
+                   const double* h_ys, double endurance_mpa, double overload_mpa, double fos_ref_mpa,
+                   double* mean, double* alt, double* eq, double* life, double* damage, double* fos,
+                   hipStream_t st);
 
 }  // namespace vox8

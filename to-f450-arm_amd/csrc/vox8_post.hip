@@ -86,6 +86,57 @@ k_stress_inv(int64_t n, const double* __restrict__ s, double* __restrict__ vm,
   }
 }
 
+// PA12 SLS S-N curve at R = -1 (src/fatigue.py:20-21): life table in log-log
+// space, stress abscissae increasing (the repeated 12 MPa plateau value is
+// separated by 1e-12, as the reference's interpolation table does).
+struct SnTable {
+  double xs[7], ys[7];   // log10(MPa), log10(cycles), ascending stress
+  double endurance_mpa, overload_mpa, fos_ref_mpa;
+};
+
+// life of an equivalent alternating stress (src/fatigue.py:24-41): linear
+// interpolation of log10(life) in log10(stress), clamped at the table ends;
+// below 0.99x the endurance stress 1e9 cycles, above 1.5x the top stress 1.
+__device__ double sn_life_dev(const SnTable& t, double s_pa) {
+  const double s = s_pa / 1e6;
+  const double ls = log10(s > 0.0 ? s : 1e-10);
+  double ly;
+  if (ls <= t.xs[0]) {
+    ly = t.ys[0];
+  } else if (ls >= t.xs[6]) {
+    ly = t.ys[6];
+  } else {
+    int k = 0;
+    while (k < 5 && ls >= t.xs[k + 1]) ++k;
+    ly = t.ys[k] + (ls - t.xs[k]) * (t.ys[k + 1] - t.ys[k]) / (t.xs[k + 1] - t.xs[k]);
+  }
+  double life = pow(10.0, ly);
+  if (s < t.endurance_mpa * 0.99) life = 1e9;
+  if (s > t.overload_mpa * 1.5) life = 1.0;
+  return life;
+}
+
+// Goodman / S-N field of two load-case stress fields (src/fatigue.py:44-121).
+__global__ void __launch_bounds__(256)
+k_fatigue(int64_t n, SnTable t, const double* __restrict__ s1, const double* __restrict__ s2,
+          double uts, double* __restrict__ mean, double* __restrict__ alt, double* __restrict__ eq,
+          double* __restrict__ life, double* __restrict__ damage, double* __restrict__ fos) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = s1[i], b = s2[i];
+    const double peak = fmax(a, b), valley = fmin(a, b);
+    const double m = 0.5 * (peak + valley), al = 0.5 * (peak - valley);
+    const double e = al / fmax(1.0 - fmax(m, 0.0) / uts, 1e-6);
+    const double L = sn_life_dev(t, e);
+    if (mean) mean[i] = m;
+    if (alt) alt[i] = al;
+    if (eq) eq[i] = e;
+    if (life) life[i] = L;
+    if (damage) damage[i] = 1.0 / fmax(L, 1.0);
+    if (fos) fos[i] = t.fos_ref_mpa * 1e6 / fmax(e, 1e-6);
+  }
+}
+
 static unsigned blocks_for(int64_t n) {
   const int64_t b = (n + 255) / 256;
   return (unsigned)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
@@ -101,6 +152,22 @@ int launch_stress(const Grid& g, const StressCoef& sc, const double* u, const do
 int launch_stress_invariants(int64_t n, const double* s, double* vm, double* pmax, hipStream_t st) {
   if (n <= 0) return OK;
   hipLaunchKernelGGL(k_stress_inv, dim3(blocks_for(n)), dim3(256), 0, st, n, s, vm, pmax);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+int launch_fatigue(int64_t n, const double* s1, const double* s2, double uts, const double* h_xs,
+                   const double* h_ys, double endurance_mpa, double overload_mpa, double fos_ref_mpa,
+                   double* mean, double* alt, double* eq, double* life, double* damage, double* fos,
+                   hipStream_t st) {
+  if (n <= 0) return OK;
+  SnTable t;
+  for (int i = 0; i < 7; ++i) { t.xs[i] = h_xs[i]; t.ys[i] = h_ys[i]; }
+  t.endurance_mpa = endurance_mpa;
+  t.overload_mpa = overload_mpa;
+  t.fos_ref_mpa = fos_ref_mpa;
+  hipLaunchKernelGGL(k_fatigue, dim3(blocks_for(n)), dim3(256), 0, st, n, t, s1, s2, uts, mean, alt,
+                     eq, life, damage, fos);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

@@ -44,6 +44,8 @@ SIGNATURES = {
     "vox8_set_mg_graphs": [_vp, _i32],
     "vox8_stress": [_vp, _vp, _vp, _pf64, _pf64, _vp],
     "vox8_stress_invariants": [_vp, C.c_int64, _vp, _vp, _vp],
+    "vox8_fatigue": [_vp, C.c_int64, _vp, _vp, _f64, _pf64, _pf64, _f64, _f64, _f64,
+                     _vp, _vp, _vp, _vp, _vp, _vp],
     "vox8_sensitivities": [_vp, _vp, _i32, _i64, _pf64, _vp, _f64, _vp, _f64, _f64, _f64,
                            _f64, _vp, _vp, _vp, _pf64, _pf64],
     "vox8_filter_setup": [_vp, _f64, _vp],
