@@ -607,19 +607,21 @@ static int tile_rows(const Grid& g) {
   return tiles_y(g, 12) * 12 < tiles_y(g, 8) * 8 ? 12 : 8;
 }
 
-// Node planes per workgroup: halve the z range until the launch has 384
-// workgroups per column.  Grids with at least that many (tile, plane) pairs
-// keep chunks of >= 4 planes (halo work stays modest; the fine grid gets
-// ~7-plane chunks); smaller grids, the coarse multigrid levels, whose kernels
-// are bound by the latency of the z march, go down to single planes.
-// VOX8_ZCHUNK fixes the chunk.
-static int fast_zchunk(const Grid& g, int ty) {
+// Node planes per workgroup (the fixed rule; FP32 launches are autotuned
+// instead, see below): halve the z range until the launch has 384 workgroups
+// over all columns.  Grids with at least that many (tile, plane) pairs keep
+// chunks of >= 4 planes (halo work stays modest; one column of the c5t grid
+// gets 7-plane chunks, six columns the whole z range, both measured best);
+// smaller grids, whose kernels are bound by the latency of the z march, go
+// down to single planes.  VOX8_ZCHUNK fixes the chunk.
+static int fast_zchunk(const Grid& g, int ty, int ncols) {
   static const int ov = env_int("VOX8_ZCHUNK", 0);
   if (ov > 0) return ov < g.NZ ? ov : g.NZ;
   const int64_t tiles = (int64_t)tiles_x(g) * tiles_y(g, ty);
+  const int64_t wide = tiles * (ncols > 0 ? ncols : 1);
   const int zmin = tiles * g.NZ >= 384 ? 4 : 1;
   int z = g.NZ;
-  while (z > zmin && tiles * ((g.NZ + z - 1) / z) < 384) z = (z + 1) / 2;
+  while (z > zmin && wide * ((g.NZ + z - 1) / z) < 384) z = (z + 1) / 2;
   return z;
 }
 
@@ -657,7 +659,7 @@ static StencilShape stencil_shape(const Grid& g, int ncols) {
   }
   StencilShape sh;
   sh.ty = tile_rows<T>(g);
-  sh.zc = fast_zchunk(g, sh.ty);
+  sh.zc = fast_zchunk(g, sh.ty, ncols);
   return sh;
 }
 
@@ -748,7 +750,7 @@ int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, cons
   // Candidates within 3 % of the fastest count as equal (run-to-run timing
   // noise); among those the one computing the fewest elements (least halo
   // work and traffic) wins, so the choice is stable across runs.
-  StencilShape best{tile_rows<float>(g), fast_zchunk(g, tile_rows<float>(g))};
+  StencilShape best{tile_rows<float>(g), fast_zchunk(g, tile_rows<float>(g), ncols)};
   float fastest = 1e30f;
   for (const Cand& c : done) fastest = std::min(fastest, c.ms);
   int64_t best_work = INT64_MAX;
