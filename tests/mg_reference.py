@@ -107,7 +107,7 @@ class VCycle:
         for L in self.levels[:-1]:
             A = sp.diags(L["dinv"]) @ L["K"]
             L["lam_max"] = 1.1 * float(np.max(np.abs(np.linalg.eigvals(A.toarray())))) if A.shape[0] < 3000 \
-                else 1.1 * _power(A)
+                else 1.1 * lanczos_lam_max(L["K"], L["dinv"])
         C = self.levels[-1]
         Kc = C["K"].toarray()
         Kc[~C["free"], :] = 0
@@ -147,11 +147,50 @@ class VCycle:
         return x
 
 
-def _power(A, it=60):
-    v = np.random.default_rng(0).standard_normal(A.shape[0])
-    lam = 0.0
-    for _ in range(it):
-        w = A @ v
-        lam = np.linalg.norm(w)
-        v = w / lam
-    return lam
+def hash_start(n):
+    """The device's Lanczos start vector: a fixed hash of the DOF index in [-0.5, 0.5)."""
+    i = np.arange(n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = i * np.uint64(0x9E3779B97F4A7C15) + np.uint64(0x632BE59BD9B4E019)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z ^= z >> np.uint64(31)
+    return ((z >> np.uint64(40)).astype(np.float64) / 16777216.0 - 0.5).astype(np.float32)
+
+
+def lanczos_lam_max(K, dinv, steps=20):
+    """Largest Ritz value of D^-1 K after `steps` Lanczos steps in the D inner
+    product from hash_start, FP32 vectors and FP64 sums as on the device."""
+    dinv = np.asarray(dinv, dtype=np.float32)
+    free = dinv > 0
+    d = np.where(free, 1.0 / np.where(free, dinv, 1.0), 0.0)
+    q = np.where(free, hash_start(K.shape[0]), 0.0).astype(np.float32)
+    q = (q / np.sqrt(np.sum(q.astype(np.float64) ** 2 * d))).astype(np.float32)
+    qp = np.zeros_like(q)
+    beta = 0.0
+    a, b = [], []
+    for _ in range(steps):
+        w = (dinv * (K @ q.astype(np.float64)).astype(np.float32) - np.float32(beta) * qp).astype(np.float32)
+        s1 = np.sum(w.astype(np.float64) * q * d)
+        s2 = np.sum(w.astype(np.float64) ** 2 * d)
+        s3 = np.sum(q.astype(np.float64) ** 2 * d)
+        alpha = s1 / s3
+        beta = np.sqrt(max(s2 - s1 * s1 / s3, 0.0))
+        a.append(alpha)
+        b.append(beta)
+        if not beta > 1e-7 * max(abs(x) for x in a):
+            break
+        qp, q = q, ((w - np.float32(alpha) * q) / np.float32(beta)).astype(np.float32)
+    m = len(a)
+    T = np.diag(a) + np.diag(b[:m - 1], 1) + np.diag(b[:m - 1], -1)
+    return float(np.linalg.eigvalsh(T)[-1])
+
+
+def exact_lam_max(K, dinv):
+    """lambda_max(D^-1 K) from the symmetric form D^-1/2 K D^-1/2."""
+    s = np.sqrt(np.asarray(dinv, dtype=np.float64))
+    S = (sp.diags(s) @ K @ sp.diags(s)).tocsr()
+    if S.shape[0] < 3000:
+        return float(np.linalg.eigvalsh(S.toarray())[-1])
+    import scipy.sparse.linalg as sla
+    return float(sla.eigsh(S, k=1, which="LA", tol=1e-10)[0][0])

@@ -132,3 +132,41 @@ def test_graph_replays_match_eager(gpu):
                       geom.n_dofs)
             torch.cuda.synchronize()
             assert np.array_equal(z.cpu().numpy(), ref), rep
+
+
+def _mg_lams(fem):
+    nl = C.c_int()
+    lam = (C.c_double * 16)()
+    fem._call("vox8_mg_info", C.byref(nl), None, lam, 16)
+    return [lam[l] for l in range(nl.value)]
+
+
+def test_lam_max_covers_spectrum_and_is_history_free(gpu):
+    """Device lambda_max(D^-1 K) per smoothed level: 1.1 x a Ritz value within
+    3 % below the exact top eigenvalue (so at least lambda_max itself), and a
+    function of the current density alone: the same density after another one
+    gives bitwise the same estimates (no state carried between updates, none
+    read from uninitialised work memory)."""
+    from mg_reference import VCycle, exact_lam_max
+    # NaN left in freed device memory, which the context's work vectors may
+    # reuse: the estimate must not read a work vector before writing it
+    junk = torch.full((64 << 20,), float("nan"), dtype=torch.float64, device=gpu)
+    del junk
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    geom, fem, rho, _lib = _setup((33, 9, 7), 11, 0.1)
+    lam_a = _mg_lams(fem)
+    f = np.zeros(geom.n_elements)
+    f[fem.active_elems] = fem.E_MIN_FRAC + (1 - fem.E_MIN_FRAC) * rho ** 3
+    V = VCycle((geom.nx, geom.ny, geom.nz), f, fem.Ke_solid, fem.node_fixed,
+               active=fem.elem_active_grid)
+    assert len(V.levels) == len(lam_a)
+    for l, L in enumerate(V.levels[:-1]):
+        exact = exact_lam_max(L["K"], L["dinv"])
+        assert 0.97 * 1.1 * exact <= lam_a[l] <= 1.1 * exact * (1 + 1e-4), (l, lam_a[l], exact)
+    rho_b = random_density(fem.n_active, 99, 0.05, 1.0)
+    fem._set_density(rho_b)
+    lam_b = _mg_lams(fem)
+    assert lam_b[0] != lam_a[0]
+    fem._set_density(rho.copy())
+    assert _mg_lams(fem) == lam_a

@@ -6,6 +6,15 @@
 
 namespace vox8 {
 
+// Layout of MultiGrid::scal: [MG_SCAL_NORM] start-vector scale, [7] coarse
+// pivot flag, from MG_SCAL_LZ: per smoothed level l, MG_LANCZOS_STEPS pairs
+// (alpha_j, beta_j) at MG_SCAL_LZ + 2 * MG_LANCZOS_STEPS * l + 2 j.
+constexpr int MG_LANCZOS_STEPS = 20;
+constexpr int MG_MAX_LEVELS = 16;
+constexpr int MG_SCAL_NORM = 0;
+constexpr int MG_SCAL_LZ = 8;
+constexpr int MG_SCAL_SIZE = MG_SCAL_LZ + 2 * MG_LANCZOS_STEPS * MG_MAX_LEVELS;
+
 struct MGLevel {
   Grid g{};
   double scale = 1.0;          // element matrix of this level = scale * Ke (box scaled by 2^l)
@@ -18,8 +27,7 @@ struct MGLevel {
   uint8_t* fixed = nullptr;    // per-node fixed-DOF bitmask
   float* dinv = nullptr;       // Jacobi inverse diagonal
   double lam_max = 2.0;        // estimate of lambda_max(D^-1 K)
-  float* pv = nullptr;         // power-iteration vector, kept across density updates
-  bool pv_init = false;
+  float* pv = nullptr;         // Lanczos vector (the other one is d)
   // work vectors, ncols column-major blocks of n_dofs
   float *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d2 = nullptr;
 };
@@ -52,8 +60,8 @@ struct MultiGrid {
   float* Ainv = nullptr;       // FP32 copy used in the V-cycle
   double* red = nullptr;       // reduction scratch
   unsigned int* counter = nullptr;
-  double* scal = nullptr;      // power-iteration scalars
-  double* host = nullptr;      // pinned
+  double* scal = nullptr;      // MG_SCAL_SIZE device scalars (Lanczos, coarse pivot check)
+  double* host = nullptr;      // pinned copy of scal
   float* tmp32 = nullptr;      // fine-level input/output staging (ncols * n0)
   bool built = false;
   bool updated = false;
