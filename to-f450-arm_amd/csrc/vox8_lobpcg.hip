@@ -8,14 +8,15 @@
 //   * a final explicit Rayleigh-Ritz on the returned block (as scipy does).
 // Design (written for this hardware, not a transcription of scipy):
 //   * basis S = [X | W | P] and the products AS = K S, BS = M S are kept as
-//     column-major blocks; only W is multiplied by the fused K&M stencil per
-//     iteration, AX/AP/BX/BP follow from the Ritz coefficients;
+//     column-major blocks; every product is formed by the fused K&M stencil
+//     (W, then X' and P' after the Ritz update), none is updated implicitly;
 //   * the Rayleigh-Ritz problem is solved on the device in FP64 by one
 //     workgroup: diagonal scaling, SVQB deflation of the Gram matrix of M
 //     (Stathopoulos & Wu 2002) and cyclic Jacobi eigen-solves with a
 //     round-robin parallel ordering (Golub & Van Loan, sec. 8.5);
 //   * the Ritz update, residual, its norms and the preconditioned residual
 //     are one pass over the blocks.
+#include <cstdio>
 #include <cstdlib>
 
 #include "vox8_kernels.h"
@@ -365,16 +366,16 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
 }
 
 // ---- Ritz update (templated on the block size) ------------------------------
-// S/AS/BS: column-major blocks (ld n) holding [X | W | P]; the first NB*K
-// columns form the current basis.  Writes X' = S C into the X slots of S and,
-// for NB > 1, the new directions P' = [W P] C[K:] into the P slots of S, AS
-// and BS.  AX' and BX' are not formed here: they are recomputed exactly by a
-// stencil pass (implicit updates drift by ~eps * lambda_max per iteration,
-// which swamps the residual of the lowest modes).
+// S: column-major blocks (ld n) holding [X | W | P]; the first NB*K columns
+// form the current basis.  Writes X' = S C into the X slots and, for NB > 1,
+// the new directions P' = [W P] C[K:] into the P slots.  No product is
+// updated here: K X', M X', K P', M P' are recomputed exactly by the stencil.
+// Implicit updates (K P' = [KW KP] C[K:]) carry the rounding of every earlier
+// step, amplified by the cancellation in P' near convergence; the error grows
+// geometrically and the Rayleigh-Ritz then picks diverging combinations.
 template <int K, int NB>
 __global__ void __launch_bounds__(256)
-k_ritz_combine(int64_t n, double* __restrict__ S, double* __restrict__ AS,
-               double* __restrict__ BS, const double* __restrict__ Cg) {
+k_ritz_combine(int64_t n, double* __restrict__ S, const double* __restrict__ Cg) {
   constexpr int M = NB * K;
   __shared__ double C[M * K];
   for (int e = threadIdx.x; e < M * K; e += blockDim.x) C[e] = Cg[e];
@@ -393,22 +394,6 @@ k_ritz_combine(int64_t n, double* __restrict__ S, double* __restrict__ AS,
       for (int i = K; i < M; ++i) p += v[i] * C[i * K + j];
       S[(int64_t)j * n + r] = x + p;
       if (NB > 1) S[(int64_t)(2 * K + j) * n + r] = p;
-    }
-    if (NB > 1) {
-#pragma unroll
-      for (int blk = 0; blk < 2; ++blk) {
-        double* Z = blk == 0 ? AS : BS;
-        double w[M > K ? M - K : 1];
-#pragma unroll
-        for (int i = K; i < M; ++i) w[i - K] = Z[(int64_t)i * n + r];
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-          double p = 0.0;
-#pragma unroll
-          for (int i = K; i < M; ++i) p += w[i - K] * C[i * K + j];
-          Z[(int64_t)(2 * K + j) * n + r] = p;
-        }
-      }
     }
   }
 }
@@ -495,9 +480,9 @@ static int gram_and_rr(LobpcgWork& w, int m, int k, hipStream_t st, double drop_
 template <int K>
 static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStream_t st) {
   const dim3 g(grid_blocks(w.n)), b(256);
-  if (nb == 1) hipLaunchKernelGGL((k_ritz_combine<K, 1>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
-  else if (nb == 2) hipLaunchKernelGGL((k_ritz_combine<K, 2>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
-  else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, w.AS, w.BS, C);
+  if (nb == 1) hipLaunchKernelGGL((k_ritz_combine<K, 1>), g, b, 0, st, w.n, w.S, C);
+  else if (nb == 2) hipLaunchKernelGGL((k_ritz_combine<K, 2>), g, b, 0, st, w.n, w.S, C);
+  else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, C);
 }
 
 // Soft locking: a pair whose residual norm fell below tol (sticky, as the
@@ -563,8 +548,9 @@ static void launch_residual(const LobpcgWork& w, const double* lam, const double
     case 11: F<11>(__VA_ARGS__); break; default: F<12>(__VA_ARGS__); break;                 \
   }
 
-// X' (and P') from the Ritz coefficients, exact AX'/BX' by the stencil,
-// residual; tol2 > 0 also soft-locks converged pairs.
+// X' (and P') from the Ritz coefficients, their exact K and M products by
+// the stencil, residual; tol2 > 0 also soft-locks converged pairs.  Pairs
+// locked earlier skip the P products (their P, KP, MP are zeroed by the lock).
 static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
                      const ApplyKM& applyKM, hipStream_t st, double tol2) {
   const double* C = w.G + 2 * MMAX * MMAX;
@@ -572,6 +558,10 @@ static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
   VOX8_K_SWITCH(k, launch_combine, w, nb, C, st);
   VOX8_HIP(hipGetLastError());
   VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st, nullptr));
+  if (nb > 1) {
+    const int64_t off = (int64_t)2 * k * w.n;
+    VOX8_TRY(applyKM(w.S + off, w.AS + off, w.BS + off, k, st, w.conv));
+  }
   VOX8_K_SWITCH(k, launch_residual, w, lam, dinv, st);
   VOX8_HIP(hipGetLastError());
   if (tol2 > 0.0) {
@@ -647,6 +637,21 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
   bool all = update_conv();
   int it = 0;
   bool have_p = false;
+  // VOX8_LOBPCG_LOG=1: one stderr line per iteration (status, lam, residual
+  // norms, sticky convergence flags), from the state read back anyway
+  static const bool log = [] {
+    const char* e = getenv("VOX8_LOBPCG_LOG");
+    return e && e[0] != '0';
+  }();
+  auto trace = [&]() {
+    if (!log) return;
+    fprintf(stderr, "[vox8 lobpcg] it %d status %g |", it, w.host[12]);
+    for (int j = 0; j < k; ++j) fprintf(stderr, " %.10e", w.host[j]);
+    fprintf(stderr, " |");
+    for (int j = 0; j < k; ++j) fprintf(stderr, " %.2e%s", sqrt(fabs(w.host[13 + j])), j < kw && conv[j] ? "*" : "");
+    fprintf(stderr, "\n");
+  };
+  trace();
   while (!all && it < maxiter) {
     VOX8_TRY(applyKM(W, AW, BW, k, st, w.conv));   // locked pairs: W, KW, MW stay zero
     const int m = have_p ? 3 * k : 2 * k;
@@ -659,6 +664,7 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
     VOX8_TRY(read_state(w, k, st));
     if (w.host[12] != 0.0) break;           // basis collapsed: X was kept, stop here
     all = update_conv();
+    trace();
   }
   // final Rayleigh-Ritz on X alone (scipy lobpcg post-processing); AX, BX
   // are exact products of the current X
