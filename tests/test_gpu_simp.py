@@ -34,3 +34,24 @@ def test_simp_loop_matches_reference_run(gpu, prec):
     errs["u"] = rel(np.stack(res["u_per_lc"]), gold["u_per_lc"])
     print(prec, {k: f"{v:.1e}" for k, v in errs.items()})
     assert all(v < 1e-6 for v in errs.values()), errs
+
+
+def test_concurrent_modal_solve_matches_sequential(gpu):
+    """The modal solve in a second context on its own stream and thread,
+    overlapping the static solve, reproduces the history of running the two
+    solves one after the other in one context (deterministic kernels, no
+    shared state; launch shapes are tuned per grid, so FP32 multigrid rounding
+    may differ at the 1e-12 level)."""
+    from vox8.fem import FEM3D, Material
+    from vox8.topopt import SIMPOptimizer
+    geom, loads, opts, mat = tipbox_problem()
+    runs = {}
+    for conc in (False, True):
+        fem = FEM3D(geom, Material(**mat), precision="mixed")
+        runs[conc] = SIMPOptimizer(geom, fem, loads, solve_rtol=TIPBOX_SOLVE_RTOL,
+                                   modal_tol=TIPBOX_MODAL_TOL, modal_maxiter=TIPBOX_MODAL_MAXITER,
+                                   concurrent_solves=conc, **opts).run(verbose=False)
+    a, b = runs[False]["history"], runs[True]["history"]
+    for k in HIST + ("pcg_iters", "lobpcg_iters"):
+        assert rel(np.asarray(b[k], dtype=float), np.asarray(a[k], dtype=float)) < 1e-10, k
+    assert rel(runs[True]["rho_active"], runs[False]["rho_active"]) < 1e-10

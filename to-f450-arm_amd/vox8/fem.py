@@ -347,6 +347,18 @@ class FEM3D:
         return self.apply_BCs_and_solve(self.assemble_K(rho_active), F, method=method, x0=x0)
 
     # ------------------------------------------------------------------ modal
+    def twin(self):
+        """A second context for the same model (geometry, material, precision,
+        element matrices, SIMP constants) on the caller's current stream, e.g.
+        to run the modal solve concurrently with the static one."""
+        t = FEM3D(self.geom, self.mat, precision=self.precision)
+        t.Ke_solid = np.array(self.Ke_solid, copy=True)
+        t.Me_solid = np.array(self.Me_solid, copy=True)
+        for k in ("SIMP_PENALTY", "E_MIN_FRAC", "MASS_THRESHOLD", "MASS_PENALTY_Q"):
+            if k in self.__dict__:
+                setattr(t, k, getattr(self, k))
+        return t
+
     def modal_block(self, rho_active, n_modes=6, tip_mass_kg=0.060, tip_mass_nodes=None,
                     X0=None, tol=1e-4, maxiter=500, seed=0, precision=None):
         """Device-resident modal solve: returns (lam (k,) numpy, X (k, n_dofs) torch, info)."""

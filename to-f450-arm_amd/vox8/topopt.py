@@ -6,11 +6,18 @@ e = ex + nx (ey + ny ez); the host sees only scalars (compliance, volume,
 change, eigenvalues, MAC) that drive the loop control.  Kernels:
 cone filter, adjoint sensitivities, OC bisection and the reductions are
 libvox8 entry points; the static solve is one batched PCG over all load cases
-and the modal solve is LOBPCG warm-started from the previous iteration.
+and the modal solve is LOBPCG warm-started from the previous iteration.  The
+two solves depend only on the physical density, so the modal one runs in a
+second vox8 context on its own stream and host thread, concurrently with the
+static one: both spend much of their time in latency-bound coarse multigrid
+levels that leave most of the GPU idle.  Every kernel is deterministic and
+the two contexts share no state, so the results are bitwise those of running
+the solves one after the other.
 """
 from __future__ import annotations
 
 import time
+from concurrent.futures import Future, ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -95,14 +102,15 @@ class SIMPOptimizer:
     Extra keyword arguments (defaults reproduce the reference):
     F_list (n_lc load vectors, instead of geometry load dicts), solve_rtol (None =
     the reference's effective 1e-5, src/fem.py:558-561), modal_tol / modal_maxiter
-    (reference loop defaults 1e-4 / 500), solve_maxiter (5000).
+    (reference loop defaults 1e-4 / 500), solve_maxiter (5000), concurrent_solves
+    (True: the modal solve overlaps the static one in a second context).
     """
 
     def __init__(self, geom, fem: FEM3D, load_cases, vol_frac=0.30, r_min_mm=2.0, move=0.2,
                  max_iter=100, tol_change=0.01, weights=None, min_freq_Hz=None, tip_mass_kg=0.060,
                  n_modes_track=4, mu_freq_init=0.10, mu_freq_max=50.0, mu_freq_ramp_every=5,
                  mu_freq_ramp_factor=1.5, F_list=None, solve_rtol=None, modal_tol=1e-4,
-                 modal_maxiter=500, solve_maxiter=5000):
+                 modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True):
         self.geom, self.fem, self.load_cases = geom, fem, load_cases
         self.vol_frac = vol_frac
         self.r_min_vox = max(1.0, r_min_mm / (geom.dx * 1e3))
@@ -115,6 +123,8 @@ class SIMPOptimizer:
         self.solve_rtol = 1e-5 if solve_rtol is None else solve_rtol
         self.solve_maxiter = solve_maxiter
         self.modal_tol, self.modal_maxiter = modal_tol, modal_maxiter
+        self.concurrent_solves = bool(concurrent_solves)
+        self._fem_modal = None
         if F_list is None:
             F_list = []
             for lc in load_cases:
@@ -179,6 +189,16 @@ class SIMPOptimizer:
         self.X = None
         self.v_prev = None
         self.modal = self.min_freq_Hz is not None
+        if self.modal and self.concurrent_solves and self._fem_modal is None:
+            # the modal solve runs in a second context on its own stream and
+            # host thread, concurrently with the static solve (independent
+            # given the density; their latency-bound coarse-grid phases leave
+            # most of the GPU idle)
+            self._modal_stream = torch.cuda.Stream(device=dev)
+            with torch.cuda.stream(self._modal_stream):
+                self._fem_modal = fem.twin()
+            self._pool = ThreadPoolExecutor(max_workers=1)
+            self._overlap = False
         self.tip = _tip_nodes_of(geom) if self.modal else np.array([], dtype=np.int64)
         self.w_target = (2.0 * np.pi * self.min_freq_Hz) ** 2 if self.modal else None
         self.history = {k: [] for k in ("compliance", "mass", "change", "vol_frac", "omega_1_Hz",
@@ -198,6 +218,32 @@ class SIMPOptimizer:
                   ptr(self._phys))
         rho_phys = fem.active_from_grid(self._phys)
         self.rho_phys = rho_phys
+        modal_job = None
+        if self.modal and self._fem_modal is not None:
+            ready = torch.cuda.Event()
+            ready.record()
+            X0 = self.X
+
+            def run_modal():
+                torch.cuda.set_device(fem.device)
+                with torch.cuda.stream(self._modal_stream):
+                    self._modal_stream.wait_event(ready)
+                    res = self._fem_modal.modal_block(rho_phys, self.n_modes_track, self.tip_mass_kg,
+                                                      self.tip, X0=X0, tol=self.modal_tol,
+                                                      maxiter=self.modal_maxiter)
+                    done = torch.cuda.Event()
+                    done.record()
+                return res, done
+
+            if self._overlap:
+                modal_job = self._pool.submit(run_modal)
+            else:
+                # first step: the solves run one after the other, so the
+                # stencil launch-shape autotuner times each new grid and
+                # width on an otherwise idle GPU; later steps overlap
+                modal_job = Future()
+                modal_job.set_result(run_modal())
+                self._overlap = True
         self.U, pinfo = fem.solve_block(rho_phys, self.F, x0=self.U, rtol=self.solve_rtol,
                                         maxiter=self.solve_maxiter)
         s = fem.sensitivities(rho_phys, U=self.U, weights=self.weights)
@@ -205,8 +251,15 @@ class SIMPOptimizer:
         self._filter(s["dc"], self._sens_f)
         lob_iters = 0
         if self.modal:
-            lam, X, linfo = fem.modal_block(rho_phys, self.n_modes_track, self.tip_mass_kg, self.tip,
-                                            X0=self.X, tol=self.modal_tol, maxiter=self.modal_maxiter)
+            if modal_job is not None:
+                (lam, X, linfo), done = modal_job.result()
+                cur = torch.cuda.current_stream()
+                cur.wait_event(done)
+                X.record_stream(cur)
+            else:
+                lam, X, linfo = fem.modal_block(rho_phys, self.n_modes_track, self.tip_mass_kg,
+                                                self.tip, X0=self.X, tol=self.modal_tol,
+                                                maxiter=self.modal_maxiter)
             lob_iters = linfo["iters"]
             order = np.argsort(lam)
             lam = np.maximum(lam[order], 0.0)
