@@ -206,6 +206,39 @@ int vox8_axpby(vox8_ctx* ctx, int64_t n, const uint8_t* d_mask, double a, const 
 int vox8_compose_density(vox8_ctx* ctx, const uint8_t* d_design, const uint8_t* d_in_arm,
                          const double* d_filt, double* d_out);
 
+/* Device-resident tail of one SIMP iteration (src/topopt.py:327-399; replaces
+ * the host-side mode tracking, penalty scaling and bookkeeping between
+ * vox8_block_dots, vox8_sensitivities, vox8_filter, vox8_stats, vox8_axpby,
+ * vox8_oc_update calls).  With k > 0 modes (columns of d_X, sorted by the
+ * eigenvalues h_lam, clipped at 0): MAC tracking against d_vprev (have_prev;
+ * else the lowest mode), the tracked mode copied into d_vprev, violation
+ * max(0, 1 - lam/w_target); if positive, the eigenvalue sensitivity of the
+ * tracked mode (tip mass as set) into d_dlam, filtered into d_dlam_f, and
+ * d_sens_f += -mu viol (mean|sens_f| / max(mean|dlam_f|, 1e-30)) d_dlam_f over
+ * the design.  Then the OC update of d_x into d_xnew, its filtered field into
+ * d_check, and h_state[20] = {tracked mode, lam, violation, stats of sens_f
+ * (4), of dlam_f (4), of d_check - d_phys_prev (4: [13] = change), of d_xnew
+ * on the design (4: [15] = volume sum), penalty coefficient}.  One host
+ * synchronisation. */
+typedef struct vox8_tail_args {
+  const double* d_X;         /* k column-major modes (ld ldx), sorted */
+  const double* h_lam;       /* k eigenvalues, ascending, >= 0 */
+  double* d_vprev;           /* n_dofs: previous tracked mode in, tracked mode out */
+  const double* d_rho;       /* element-grid physical density */
+  double* d_dlam;            /* n_elem scratch: eigenvalue sensitivity */
+  double* d_dlam_f;          /* n_elem scratch: its filtered field */
+  double* d_sens_f;          /* n_elem: filtered compliance sensitivity (updated) */
+  const uint8_t* d_design;   /* n_elem design mask */
+  const double* d_x;         /* n_elem design variables */
+  double* d_xnew;            /* n_elem: OC result */
+  double* d_check;           /* n_elem: filtered OC result */
+  const double* d_phys_prev; /* n_elem: previous filtered design */
+  int64_t ldx;
+  double w_target, mu, p, e_min, mthr, mq, V_e, target_volume, move;
+  int k, have_prev;
+} vox8_tail_args;
+int vox8_simp_tail(vox8_ctx* ctx, const vox8_tail_args* args, double* h_state);
+
 #ifdef __cplusplus
 }
 #endif

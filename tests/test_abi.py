@@ -17,3 +17,25 @@ def test_library_exports_header_symbols():
     assert not missing, missing
     assert set(_lib.SIGNATURES) | {"vox8_last_error"} == declared
     assert lib.vox8_version() == 100
+
+
+def test_tail_args_layout_matches_header(tmp_path):
+    """ctypes TailArgs has the offsets and size of struct vox8_tail_args
+    (compiled with gcc against include/vox8.h)."""
+    import shutil
+    import subprocess
+    import pytest
+    from vox8 import _lib
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    names = [f[0] for f in _lib.TailArgs._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"vox8.h\"\nint main(void) {\n"
+                   + "".join(f'  printf("%zu\\n", offsetof(vox8_tail_args, {n}));\n' for n in names)
+                   + '  printf("%zu\\n", sizeof(vox8_tail_args));\n  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [getattr(_lib.TailArgs, n).offset for n in names] + [ctypes.sizeof(_lib.TailArgs)]
+    assert got == want

@@ -55,3 +55,38 @@ def test_concurrent_modal_solve_matches_sequential(gpu):
     for k in HIST + ("pcg_iters", "lobpcg_iters"):
         assert rel(np.asarray(b[k], dtype=float), np.asarray(a[k], dtype=float)) < 1e-10, k
     assert rel(runs[True]["rho_active"], runs[False]["rho_active"]) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["active", "never_binding", "no_modal"])
+def test_device_tail_matches_host_tail(gpu, case):
+    """The device-resident loop tail (vox8_simp_tail: MAC tracking, eigenvalue
+    penalty, OC, filter, bookkeeping, one host read) against the same steps
+    driven from the host: identical histories and designs.  Cases: frequency
+    constraint active (penalty applied), a target below every frequency (the
+    gated branch never runs) and no modal constraint."""
+    from vox8.fem import FEM3D, Material
+    from vox8.topopt import SIMPOptimizer
+    geom, loads, opts, mat = tipbox_problem()
+    opts = dict(opts)
+    if case == "never_binding":
+        opts["min_freq_Hz"] = 1.0
+    elif case == "no_modal":
+        opts["min_freq_Hz"] = None
+    runs = {}
+    for dev_tail in (False, True):
+        fem = FEM3D(geom, Material(**mat), precision="mixed")
+        runs[dev_tail] = SIMPOptimizer(geom, fem, loads, solve_rtol=TIPBOX_SOLVE_RTOL,
+                                       modal_tol=TIPBOX_MODAL_TOL, modal_maxiter=TIPBOX_MODAL_MAXITER,
+                                       device_tail=dev_tail, **opts).run(verbose=False)
+    a, b = runs[False]["history"], runs[True]["history"]
+    if case == "active":
+        assert max(a["freq_violation"]) > 0.0
+    if case == "never_binding":
+        assert max(a["freq_violation"]) == 0.0
+    for k in HIST:
+        x, y = np.asarray(a[k], dtype=float), np.asarray(b[k], dtype=float)
+        if case == "no_modal" and k in ("omega_1_Hz", "mu_freq", "freq_violation"):
+            assert np.all(np.isnan(x)) and np.all(np.isnan(y))
+            continue
+        assert rel(y, x) < 1e-12, (k, x, y)
+    assert rel(runs[True]["rho_active"], runs[False]["rho_active"]) < 1e-12

@@ -67,6 +67,7 @@ struct vox8_ctx {
   uint64_t density_version = 0, mg_version = ~0ull;
   // OC / sensitivities scratch
   double* d_oc = nullptr;
+  double* d_tail = nullptr;         // TAIL_STATE doubles
   double *d_kphi = nullptr, *d_mphi = nullptr, *d_ce = nullptr, *d_sums = nullptr,
          *d_wts = nullptr;
 };
@@ -292,7 +293,8 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
       (s = dalloc(c, &c->dinvK, nd)) || (s = dalloc(c, &c->dinvK32, nd)) ||
       (s = dalloc(c, &c->red_partials, 4096 * 40)) || (s = dalloc(c, &c->red_counters, 64)) ||
       (s = dalloc(c, &c->red_out, 64)) || (s = halloc(c, &c->host, 64)) ||
-      (s = dalloc(c, &c->d_oc, 8)) || (s = dalloc(c, &c->d_kphi, ne)) ||
+      (s = dalloc(c, &c->d_oc, 8)) || (s = dalloc(c, &c->d_tail, 32)) ||
+      (s = dalloc(c, &c->d_kphi, ne)) ||
       (s = dalloc(c, &c->d_mphi, ne)) || (s = dalloc(c, &c->d_ce, ne)) ||
       (s = dalloc(c, &c->d_sums, 32)) || (s = dalloc(c, &c->d_wts, 32)) ||
       (s = dalloc(c, &c->d_den, ne)) || (s = dalloc(c, &c->d_fmask, ne)))
@@ -750,6 +752,63 @@ int vox8_axpby(vox8_ctx* c, int64_t n, const uint8_t* d_mask, double a, const do
   API_BEGIN
   VOX8_CHECK(c && d_x && d_y && n > 0, "bad argument");
   return launch_axpby(n, d_mask, a, d_x, b, d_y, c->st);
+  API_END
+}
+
+int vox8_simp_tail(vox8_ctx* c, const vox8_tail_args* a, double* h_state) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(a && h_state, "null argument");
+  VOX8_CHECK(a->k >= 0 && a->k <= 12, "tail: 0..12 modes");
+  VOX8_CHECK(a->d_sens_f && a->d_design && a->d_x && a->d_xnew && a->d_check && a->d_phys_prev,
+             "tail: null element field");
+  VOX8_CHECK(a->V_e > 0.0, "V_e must be positive");
+  if (!c->have_filter) { set_error("filter not set up (vox8_filter_setup)"); return ERR_STATE; }
+  const int64_t ne = c->g.n_elem, nd = c->g.n_dofs;
+  double* state = c->d_tail;
+  TailSelect s{};
+  s.k = a->k;
+  s.have_prev = a->have_prev;
+  s.w_target = a->w_target;
+  if (a->k > 0) {
+    VOX8_CHECK(a->d_X && a->h_lam && a->d_vprev && a->d_rho && a->d_dlam && a->d_dlam_f,
+               "tail: null modal argument");
+    VOX8_CHECK(a->ldx >= nd, "tail: ldx < n_dofs");
+    VOX8_CHECK(a->w_target > 0.0, "tail: w_target must be positive");
+    for (int j = 0; j < a->k; ++j) s.lam[j] = a->h_lam[j];
+    if (a->have_prev)
+      VOX8_TRY(launch_block_dots(nd, a->k, a->d_X, a->ldx, a->d_vprev, c->red_partials,
+                                 c->red_counters, c->red_out, c->st));
+    VOX8_TRY(launch_tail_select(s, c->red_out, state, c->st));
+    VOX8_TRY(launch_gather_col(nd, a->d_X, a->ldx, state, a->d_vprev, c->st));
+    // eigenvalue sensitivity branch, gated on the violation (state[2])
+    const double* gate = state + 2;
+    VOX8_TRY(launch_quadforms(c->g, c->fK, c->fM, c->d_active, c->dKe, c->dMe, nullptr, nd, 0,
+                              c->d_wts, a->d_vprev, nullptr, c->d_kphi, c->d_mphi, c->red_partials,
+                              c->red_counters, c->d_sums, c->d_tip, c->n_tip, c->tip_m, c->st,
+                              gate));
+    VOX8_TRY(launch_sens_finish(ne, c->d_active, a->d_rho, a->p, a->e_min, a->mthr, a->mq, c->d_ce,
+                                nullptr, c->d_kphi, c->d_mphi, 0.0, c->d_sums, a->d_dlam, c->st,
+                                state + 1, gate));
+    VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask,
+                           c->d_den, a->d_dlam, a->d_dlam_f, FILTER_FWD, c->st, gate));
+    VOX8_TRY(launch_stats(ne, a->d_design, a->d_sens_f, nullptr, c->red_partials, c->red_counters,
+                          state + 3, c->st, gate));
+    VOX8_TRY(launch_stats(ne, a->d_design, a->d_dlam_f, nullptr, c->red_partials, c->red_counters,
+                          state + 7, c->st, gate));
+    VOX8_TRY(launch_tail_penalty(ne, a->mu, state, a->d_dlam_f, a->d_sens_f, c->st));
+  } else {
+    VOX8_TRY(launch_tail_select(s, nullptr, state, c->st));
+  }
+  VOX8_TRY(launch_oc(ne, a->d_design, a->d_x, a->d_sens_f, a->V_e, a->target_volume, a->move,
+                     c->d_oc, c->red_partials, c->red_counters, a->d_xnew, 100, c->st));
+  VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask, c->d_den,
+                         a->d_xnew, a->d_check, FILTER_FWD, c->st));
+  VOX8_TRY(launch_stats(ne, nullptr, a->d_check, a->d_phys_prev, c->red_partials, c->red_counters,
+                        state + 11, c->st));
+  VOX8_TRY(launch_stats(ne, a->d_design, a->d_xnew, nullptr, c->red_partials, c->red_counters,
+                        state + 15, c->st));
+  return sync_read(c, state, h_state, TAIL_STATE);
   API_END
 }
 

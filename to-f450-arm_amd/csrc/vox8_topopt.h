@@ -1,4 +1,6 @@
 // vox8 — launchers of the SIMP-loop kernels (vox8_topopt.hip).
+// `gate` (optional, device): the launch does nothing unless *gate > 0, so a
+// launch sequence can skip a branch decided on the device without a host read.
 #pragma once
 #include "vox8_kernels.h"
 
@@ -6,25 +8,39 @@ namespace vox8 {
 enum FilterMode { FILTER_FWD = 0, FILTER_ADJ = 1, FILTER_DEN = 2 };
 int launch_filter(int nx, int ny, int nz, int ntap, const int* toff, const double* tw,
                   const uint8_t* mask, const double* den, const double* in, double* out,
-                  int mode, hipStream_t st);
+                  int mode, hipStream_t st, const double* gate = nullptr);
 int launch_quadforms(const Grid& g, const double* fK, const double* fM, const uint8_t* active,
                      const double* Ke, const double* Me, const double* U, int64_t ldu, int nrhs,
                      const double* wts, const double* phi, double* ce, double* kphi,
                      double* mphi, double* partials, unsigned int* counter, double* sums,
-                     const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st);
+                     const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st,
+                     const double* gate = nullptr);
 int launch_sens_finish(int64_t n, const uint8_t* active, const double* rho, double p, double emin,
                        double mthr, double mq, const double* ce, double* dc, const double* kphi,
                        const double* mphi, double lam, const double* vmv, double* dlam,
-                       hipStream_t st);
+                       hipStream_t st, const double* lam_dev = nullptr, const double* gate = nullptr);
 int launch_oc(int64_t n, const uint8_t* design, const double* x, const double* dc, double Ve,
               double target, double move, double* state, double* partials, unsigned int* counter,
               double* xnew, int max_steps, hipStream_t st);
 int launch_stats(int64_t n, const uint8_t* mask, const double* a, const double* b, double* partials,
-                 unsigned int* counter, double* out, hipStream_t st);
+                 unsigned int* counter, double* out, hipStream_t st, const double* gate = nullptr);
 int launch_block_dots(int64_t n, int k, const double* X, int64_t ldx, const double* v,
                       double* partials, unsigned int* counter, double* out, hipStream_t st);
 int launch_axpby(int64_t n, const uint8_t* mask, double a, const double* x, double b, double* y,
                  hipStream_t st);
+// Device-resident loop tail (state layout in vox8_topopt.hip).
+constexpr int TAIL_STATE = 20;
+struct TailSelect {
+  double lam[16];     // eigenvalues, ascending, clipped at 0
+  double w_target;    // (2 pi f_target)^2
+  int k;              // number of modes (0: no modal constraint)
+  int have_prev;      // a previous tracked mode exists
+};
+int launch_tail_select(const TailSelect& s, const double* dots, double* state, hipStream_t st);
+int launch_gather_col(int64_t n, const double* X, int64_t ldx, const double* state, double* v,
+                      hipStream_t st);
+int launch_tail_penalty(int64_t n, double mu, double* state, const double* x, double* y,
+                        hipStream_t st);
 int launch_compose(int64_t n, const uint8_t* design, const uint8_t* in_arm, const double* filt,
                    double* out, hipStream_t st);
 }  // namespace vox8

@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(256)
 k_filter(int nx, int ny, int nz, int ntap, const int* __restrict__ toff,
          const double* __restrict__ tw, const uint8_t* __restrict__ mask,
          const double* __restrict__ den, const double* __restrict__ in, double* __restrict__ out,
-         int mode) {
+         int mode, const double* __restrict__ gate) {
+  if (gate && !(*gate > 0.0)) return;
   const int64_t n = (int64_t)nx * ny * nz;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -77,9 +78,9 @@ k_filter(int nx, int ny, int nz, int ntap, const int* __restrict__ toff,
 
 int launch_filter(int nx, int ny, int nz, int ntap, const int* toff, const double* tw,
                   const uint8_t* mask, const double* den, const double* in, double* out,
-                  int mode, hipStream_t st) {
+                  int mode, hipStream_t st, const double* gate) {
   hipLaunchKernelGGL(k_filter, dim3(nblk((int64_t)nx * ny * nz, 4096)), dim3(256), 0, st, nx, ny, nz,
-                     ntap, toff, tw, mask, den, in, out, mode);
+                     ntap, toff, tw, mask, den, in, out, mode, gate);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -97,7 +98,8 @@ k_quadforms(Grid g, const double* __restrict__ fK, const double* __restrict__ fM
             double* __restrict__ ce, double* __restrict__ kphi, double* __restrict__ mphi,
             double* __restrict__ partials, unsigned int* __restrict__ counter,
             double* __restrict__ sums, const int64_t* __restrict__ tip_nodes, int n_tip,
-            double tip_m) {
+            double tip_m, const double* __restrict__ gate) {
+  if (gate && !(*gate > 0.0)) return;     // uniform over the grid: the counter is untouched
   __shared__ double sh[8], sv[20];
   double acc[17];
   const int nv = nrhs + 1;
@@ -171,7 +173,10 @@ __global__ void k_sens_finish(int64_t n, const uint8_t* __restrict__ active,
                               double mthr, double mq, const double* __restrict__ ce,
                               double* __restrict__ dc, const double* __restrict__ kphi,
                               const double* __restrict__ mphi, double lam,
-                              const double* __restrict__ vmv_parts, double* __restrict__ dlam) {
+                              const double* __restrict__ vmv_parts, double* __restrict__ dlam,
+                              const double* __restrict__ lam_dev, const double* __restrict__ gate) {
+  if (gate && !(*gate > 0.0)) return;
+  if (lam_dev) lam = *lam_dev;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
     if (!active[e]) {
@@ -195,11 +200,12 @@ int launch_quadforms(const Grid& g, const double* fK, const double* fM, const ui
                      const double* Ke, const double* Me, const double* U, int64_t ldu, int nrhs,
                      const double* wts, const double* phi, double* ce, double* kphi,
                      double* mphi, double* partials, unsigned int* counter, double* sums,
-                     const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st) {
+                     const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st,
+                     const double* gate) {
   VOX8_CHECK(nrhs >= 0 && nrhs <= 16, "quadforms: at most 16 load cases");
   hipLaunchKernelGGL(k_quadforms, dim3(nblk(g.n_elem)), dim3(256), 0, st, g, fK, fM, active, Ke,
                      Me, U, ldu, nrhs, wts, phi, ce, kphi, mphi, partials, counter, sums,
-                     tip_nodes, n_tip, tip_m);
+                     tip_nodes, n_tip, tip_m, gate);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -207,9 +213,9 @@ int launch_quadforms(const Grid& g, const double* fK, const double* fM, const ui
 int launch_sens_finish(int64_t n, const uint8_t* active, const double* rho, double p, double emin,
                        double mthr, double mq, const double* ce, double* dc, const double* kphi,
                        const double* mphi, double lam, const double* vmv, double* dlam,
-                       hipStream_t st) {
+                       hipStream_t st, const double* lam_dev, const double* gate) {
   hipLaunchKernelGGL(k_sens_finish, dim3(nblk(n, 4096)), dim3(256), 0, st, n, active, rho, p, emin,
-                     mthr, mq, ce, dc, kphi, mphi, lam, vmv, dlam);
+                     mthr, mq, ce, dc, kphi, mphi, lam, vmv, dlam, lam_dev, gate);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -285,7 +291,8 @@ int launch_oc(int64_t n, const uint8_t* design, const double* x, const double* d
 __global__ void __launch_bounds__(256)
 k_stats(int64_t n, const uint8_t* __restrict__ mask, const double* __restrict__ a,
         const double* __restrict__ b, double* __restrict__ partials, unsigned int* __restrict__ counter,
-        double* __restrict__ out) {
+        double* __restrict__ out, const double* __restrict__ gate) {
+  if (gate && !(*gate > 0.0)) return;     // uniform over the grid: the counter is untouched
   __shared__ double sh[8], sv[4], smax[4];
   double s = 0.0, sa = 0.0, mx = 0.0, cnt = 0.0;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
@@ -345,9 +352,9 @@ k_stats(int64_t n, const uint8_t* __restrict__ mask, const double* __restrict__ 
 }
 
 int launch_stats(int64_t n, const uint8_t* mask, const double* a, const double* b, double* partials,
-                 unsigned int* counter, double* out, hipStream_t st) {
+                 unsigned int* counter, double* out, hipStream_t st, const double* gate) {
   hipLaunchKernelGGL(k_stats, dim3(nblk(n, 512)), dim3(256), 0, st, n, mask, a, b, partials, counter,
-                     out);
+                     out, gate);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -399,6 +406,80 @@ __global__ void k_axpby(int64_t n, const uint8_t* __restrict__ mask, double a,
 int launch_axpby(int64_t n, const uint8_t* mask, double a, const double* x, double b, double* y,
                  hipStream_t st) {
   hipLaunchKernelGGL(k_axpby, dim3(nblk(n, 4096)), dim3(256), 0, st, n, mask, a, x, b, y);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+// ---- device-resident loop tail -------------------------------------------------
+// state layout (doubles): [0] tracked mode (sorted position), [1] its eigenvalue,
+// [2] violation, [3..6] stats of the filtered compliance sensitivity on the
+// design, [7..10] stats of the filtered eigenvalue sensitivity, [11..14] stats
+// of the change, [15..18] stats of the new design, [19] penalty coefficient.
+
+// Mode tracking by MAC against the previous tracked mode (src/topopt.py:178-188,
+// 327-339): dots = {X_j.v, X_j.X_j, v.v} of the eigenvalue-sorted modes; the
+// first maximum wins (numpy argmax); no previous mode -> the lowest.
+__global__ void k_tail_select(TailSelect s, const double* __restrict__ dots,
+                              double* __restrict__ state) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int best = 0;
+  if (s.have_prev && s.k > 0) {
+    double bm = 0.0;
+    for (int j = 0; j < s.k; ++j) {
+      const double num = dots[j];
+      const double mac = num * num / fmax(dots[s.k + j] * dots[2 * s.k], 1e-24);
+      if (j == 0 || mac > bm) { bm = mac; best = j; }
+    }
+  }
+  const double lam1 = s.k > 0 ? s.lam[best] : 0.0;
+  for (int i = 3; i < TAIL_STATE; ++i) state[i] = 0.0;   // gated stages leave zeros
+  state[0] = (double)best;
+  state[1] = lam1;
+  state[2] = s.k > 0 ? fmax(0.0, 1.0 - lam1 / s.w_target) : 0.0;
+}
+
+// v = X[:, state[0]] (the tracked mode becomes the next step's reference)
+__global__ void k_gather_col(int64_t n, const double* __restrict__ X, int64_t ldx,
+                             const double* __restrict__ state, double* __restrict__ v) {
+  const int64_t col = (int64_t)state[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = X[col * ldx + i];
+}
+
+// Augmented-Lagrangian frequency penalty (src/topopt.py:339-361): when the
+// violation is positive, y += a x with a = -mu viol (s_c / max(s_m, 1e-30)),
+// s_c and s_m the mean |.| of the filtered compliance and eigenvalue
+// sensitivities over the design (both divided by the design count).
+__global__ void k_tail_penalty(int64_t n, double mu, double* __restrict__ state,
+                               const double* __restrict__ x, double b, double* __restrict__ y) {
+  const double viol = state[2];
+  if (!(viol > 0.0)) return;
+  const double cnt = state[6];
+  const double s_c = state[4] / cnt, s_m = state[8] / cnt;
+  const double a = -mu * viol * (s_c / fmax(s_m, 1e-30));
+  if (blockIdx.x == 0 && threadIdx.x == 0) state[19] = a;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = a * x[e] + b * y[e];
+}
+
+int launch_tail_select(const TailSelect& s, const double* dots, double* state, hipStream_t st) {
+  hipLaunchKernelGGL(k_tail_select, dim3(1), dim3(64), 0, st, s, dots, state);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+int launch_gather_col(int64_t n, const double* X, int64_t ldx, const double* state, double* v,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_col, dim3(nblk(n, 4096)), dim3(256), 0, st, n, X, ldx, state, v);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+int launch_tail_penalty(int64_t n, double mu, double* state, const double* x, double* y,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_tail_penalty, dim3(nblk(n, 4096)), dim3(256), 0, st, n, mu, state, x, 1.0, y);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

@@ -20,6 +20,18 @@ _f64 = C.c_double
 _pi32 = C.POINTER(C.c_int)
 _pf64 = C.POINTER(C.c_double)
 
+class TailArgs(C.Structure):
+    """struct vox8_tail_args (include/vox8.h), field for field."""
+    _fields_ = [("d_X", _vp), ("h_lam", _pf64), ("d_vprev", _vp), ("d_rho", _vp),
+                ("d_dlam", _vp), ("d_dlam_f", _vp), ("d_sens_f", _vp), ("d_design", _vp),
+                ("d_x", _vp), ("d_xnew", _vp), ("d_check", _vp), ("d_phys_prev", _vp),
+                ("ldx", _i64), ("w_target", _f64), ("mu", _f64), ("p", _f64), ("e_min", _f64),
+                ("mthr", _f64), ("mq", _f64), ("V_e", _f64), ("target_volume", _f64),
+                ("move", _f64), ("k", _i32), ("have_prev", _i32)]
+
+
+TAIL_STATE = 20
+
 # name -> argtypes (all return int status, except vox8_last_error)
 SIGNATURES = {
     "vox8_version": [],
@@ -56,6 +68,7 @@ SIGNATURES = {
     "vox8_block_dots": [_vp, _i64, _i32, _vp, _i64, _vp, _pf64],
     "vox8_axpby": [_vp, _i64, _vp, _f64, _vp, _f64, _vp],
     "vox8_compose_density": [_vp, _vp, _vp, _vp, _vp],
+    "vox8_simp_tail": [_vp, C.POINTER(TailArgs), _pf64],
 }
 
 

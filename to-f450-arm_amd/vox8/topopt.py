@@ -103,14 +103,16 @@ class SIMPOptimizer:
     F_list (n_lc load vectors, instead of geometry load dicts), solve_rtol (None =
     the reference's effective 1e-5, src/fem.py:558-561), modal_tol / modal_maxiter
     (reference loop defaults 1e-4 / 500), solve_maxiter (5000), concurrent_solves
-    (True: the modal solve overlaps the static one in a second context).
+    (True: the modal solve overlaps the static one in a second context),
+    device_tail (True: mode tracking, penalty, OC and bookkeeping in one device
+    call with one host read; False: the same steps driven from the host).
     """
 
     def __init__(self, geom, fem: FEM3D, load_cases, vol_frac=0.30, r_min_mm=2.0, move=0.2,
                  max_iter=100, tol_change=0.01, weights=None, min_freq_Hz=None, tip_mass_kg=0.060,
                  n_modes_track=4, mu_freq_init=0.10, mu_freq_max=50.0, mu_freq_ramp_every=5,
                  mu_freq_ramp_factor=1.5, F_list=None, solve_rtol=None, modal_tol=1e-4,
-                 modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True):
+                 modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True, device_tail=True):
         self.geom, self.fem, self.load_cases = geom, fem, load_cases
         self.vol_frac = vol_frac
         self.r_min_vox = max(1.0, r_min_mm / (geom.dx * 1e3))
@@ -124,6 +126,7 @@ class SIMPOptimizer:
         self.solve_maxiter = solve_maxiter
         self.modal_tol, self.modal_maxiter = modal_tol, modal_maxiter
         self.concurrent_solves = bool(concurrent_solves)
+        self.device_tail = bool(device_tail)
         self._fem_modal = None
         if F_list is None:
             F_list = []
@@ -183,6 +186,8 @@ class SIMPOptimizer:
         self._phys = torch.empty(ne, dtype=torch.float64, device=dev)
         self._sens_f = torch.empty(ne, dtype=torch.float64, device=dev)
         self._dlam_f = torch.empty(ne, dtype=torch.float64, device=dev)
+        self._dlam = torch.zeros(ne, dtype=torch.float64, device=dev)
+        self._vprev = torch.zeros(geom.n_dofs, dtype=torch.float64, device=dev)
         self._xnew = torch.empty(ne, dtype=torch.float64, device=dev)
         self._check = torch.empty(ne, dtype=torch.float64, device=dev)
         self.U = None
@@ -250,6 +255,7 @@ class SIMPOptimizer:
         c_total = float(np.dot(self.weights, s["comp"]))
         self._filter(s["dc"], self._sens_f)
         lob_iters = 0
+        lam = X = None
         if self.modal:
             if modal_job is not None:
                 (lam, X, linfo), done = modal_job.result()
@@ -266,6 +272,67 @@ class SIMPOptimizer:
             if not np.array_equal(order, np.arange(len(order))):
                 X = X[torch.as_tensor(order, device=fem.device)].contiguous()
             self.X = X
+        if self.device_tail:
+            change, vol = self._device_tail(lam, X, it)
+        else:
+            change, vol = self._host_tail(lam, X, rho_phys, it)
+        self.x, self._xnew = self._xnew, self.x
+        self.phys_prev, self._check = self._check, self.phys_prev
+        m = vol * self.V_e
+        h = self.history
+        h["compliance"].append(c_total)
+        h["mass"].append(m)
+        h["change"].append(change)
+        h["vol_frac"].append(m / self.design_volume)
+        h["pcg_iters"].append(list(pinfo["iters"]))
+        h["lobpcg_iters"].append(lob_iters)
+        h["time_s"].append(time.perf_counter() - t0)
+        return change
+
+    def _track(self, lam1, viol, it):
+        """History and mu ramp of the frequency constraint (src/topopt.py:363-373)."""
+        self.history["omega_1_Hz"].append(float(np.sqrt(lam1) / (2.0 * np.pi)))
+        self.history["mu_freq"].append(self.mu_freq)
+        self.history["freq_violation"].append(viol)
+        if it % self.mu_freq_ramp_every == 0 and viol > 0.0 and self.mu_freq < self.mu_freq_max:
+            self.mu_freq = min(self.mu_freq_max, self.mu_freq * self.mu_freq_ramp_factor)
+
+    def _device_tail(self, lam, X, it):
+        """Mode tracking, eigenvalue-sensitivity penalty, OC update, filter and
+        bookkeeping in one device call (vox8_simp_tail): one host read per
+        iteration.  Returns (change, design volume sum)."""
+        fem = self.fem
+        a = _lib.TailArgs()
+        lam_c = None
+        if self.modal:
+            k = X.shape[0]
+            lam_c = (C.c_double * k)(*[float(v) for v in lam])
+            fem._set_tip(self.tip_mass_kg, self.tip)
+            a.d_X, a.ldx, a.h_lam = ptr(X), X.shape[1], C.cast(lam_c, C.POINTER(C.c_double))
+            a.d_vprev, a.have_prev = ptr(self._vprev), int(self.v_prev is not None)
+            a.w_target, a.mu = self.w_target, self.mu_freq
+            a.d_rho, a.d_dlam, a.d_dlam_f = ptr(fem._rho_grid), ptr(self._dlam), ptr(self._dlam_f)
+            a.p, a.e_min = fem.SIMP_PENALTY, fem.E_MIN_FRAC
+            a.mthr, a.mq = fem.MASS_THRESHOLD, fem.MASS_PENALTY_Q
+            a.k = k
+        a.d_sens_f, a.d_design, a.d_x = ptr(self._sens_f), ptr(self._design), ptr(self.x)
+        a.d_xnew, a.d_check, a.d_phys_prev = ptr(self._xnew), ptr(self._check), ptr(self.phys_prev)
+        a.V_e, a.target_volume, a.move = self.V_e, self.target_volume, self.move
+        out = (C.c_double * _lib.TAIL_STATE)()
+        fem._call("vox8_simp_tail", C.byref(a), out)
+        if self.modal:
+            self.v_prev = self._vprev
+            self._track(out[1], out[2], it)
+        else:
+            for key in ("omega_1_Hz", "mu_freq", "freq_violation"):
+                self.history[key].append(float("nan"))
+        return out[13], out[15]
+
+    def _host_tail(self, lam, X, rho_phys, it):
+        """The same steps driven from the host (one read-back per reduction),
+        the parity reference of _device_tail."""
+        fem = self.fem
+        if self.modal:
             if self.v_prev is None:
                 k = 0
             else:
@@ -278,7 +345,6 @@ class SIMPOptimizer:
                 mac = num ** 2 / np.maximum(a2 * d[2 * k_modes], 1e-24)
                 k = int(np.argmax(mac))
             lam1 = float(lam[k])
-            f1 = float(np.sqrt(lam1) / (2.0 * np.pi))
             v1 = X[k].clone()
             self.v_prev = v1
             viol = max(0.0, 1.0 - lam1 / self.w_target)
@@ -292,11 +358,7 @@ class SIMPOptimizer:
                 a = -self.mu_freq * viol * (s_c / max(s_m, 1e-30))
                 fem._call("vox8_axpby", self._sens_f.numel(), None, a, ptr(self._dlam_f), 1.0,
                           ptr(self._sens_f))
-            self.history["omega_1_Hz"].append(f1)
-            self.history["mu_freq"].append(self.mu_freq)
-            self.history["freq_violation"].append(viol)
-            if it % self.mu_freq_ramp_every == 0 and viol > 0.0 and self.mu_freq < self.mu_freq_max:
-                self.mu_freq = min(self.mu_freq_max, self.mu_freq * self.mu_freq_ramp_factor)
+            self._track(lam1, viol, it)
         else:
             for key in ("omega_1_Hz", "mu_freq", "freq_violation"):
                 self.history[key].append(float("nan"))
@@ -305,18 +367,7 @@ class SIMPOptimizer:
         self._filter(self._xnew, self._check)
         _, _, change, _ = self._stats(self._check, self.phys_prev)
         vol, _, _, _ = self._stats(self._xnew, mask=self._design)
-        self.x, self._xnew = self._xnew, self.x
-        self.phys_prev, self._check = self._check, self.phys_prev
-        m = vol * self.V_e
-        h = self.history
-        h["compliance"].append(c_total)
-        h["mass"].append(m)
-        h["change"].append(change)
-        h["vol_frac"].append(m / self.design_volume)
-        h["pcg_iters"].append(list(pinfo["iters"]))
-        h["lobpcg_iters"].append(lob_iters)
-        h["time_s"].append(time.perf_counter() - t0)
-        return change
+        return change, vol
 
     def result(self):
         fem, g = self.fem, self.geom
