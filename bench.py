@@ -127,6 +127,19 @@ def kernel_rooflines(fem, geom, rho, nrhs=6, reps=30):
     alg = 16 * n + 8 * ne + nn
     out["k_fp64"] = dict(kernel="k_apply_fast<double,CUBE,K> (FP64 K product, 1 column)", t=t, alg=alg,
                          per_unit="16 B per DOF (read x; write Kx), + 8 B/element factor, 1 B/node mask")
+    # FP32 K product, nrhs columns: the structured stencil against the MFMA
+    # trial of the 24x24 element contraction (same inputs, same output)
+    x32, y32 = d, dn
+    t = _event_time(fem, lambda: fem._call("vox8_apply", _lib.OP_K, _lib.PREC_FP32, _lib.ptr(x32),
+                                           _lib.ptr(y32), None, nrhs, n), reps)
+    alg = nrhs * 8 * n + 4 * ne + nn
+    out["k_fp32"] = dict(kernel=f"k_apply_fast<float,CUBE,K> (FP32 K product, {nrhs} columns)", t=t,
+                         alg=alg, per_unit="8 B per DOF and column (read x; write Kx), "
+                                           "+ 4 B/element factor, 1 B/node mask")
+    t = _event_time(fem, lambda: fem._call("vox8_trial_mfma_apply", _lib.ptr(x32), _lib.ptr(y32), nrhs),
+                    reps)
+    out["k_fp32_mfma_trial"] = dict(kernel=f"k_apply_mfma_trial (FP32 K product on MFMA, {nrhs} columns)",
+                                    t=t, alg=alg, per_unit="as k_fp32")
     # attainable streaming rate on this part: a 1 GiB device-to-device copy
     # (read + write bytes), timed the same way
     src = torch.ones(1 << 27, dtype=torch.float64, device=fem.device)

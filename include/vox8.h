@@ -239,6 +239,13 @@ typedef struct vox8_tail_args {
 } vox8_tail_args;
 int vox8_simp_tail(vox8_ctx* ctx, const vox8_tail_args* args, double* h_state);
 
+/* MFMA trial (no reference counterpart; a measurement, not used by the
+ * solvers): d_Y = K d_X in FP32 for nrhs column-major vectors, the 24x24
+ * element contraction on the matrix cores (v_mfma_f32_32x32x2f32) with the
+ * node assembly in LDS; zero at fixed DOFs.  Compare with vox8_apply(OP_K,
+ * PREC_FP32), the structured stencil. */
+int vox8_trial_mfma_apply(vox8_ctx* ctx, const float* d_X, float* d_Y, int nrhs);
+
 #ifdef __cplusplus
 }
 #endif

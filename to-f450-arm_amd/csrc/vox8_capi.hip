@@ -8,6 +8,7 @@
 #include "../../include/vox8.h"
 #include "vox8_fast.h"
 #include "vox8_kernels.h"
+#include "vox8_mfma_trial.h"
 #include "vox8_mg.h"
 #include "vox8_post.h"
 #include "vox8_pcg.h"
@@ -809,6 +810,14 @@ int vox8_simp_tail(vox8_ctx* c, const vox8_tail_args* a, double* h_state) {
   VOX8_TRY(launch_stats(ne, a->d_design, a->d_xnew, nullptr, c->red_partials, c->red_counters,
                         state + 15, c->st));
   return sync_read(c, state, h_state, TAIL_STATE);
+  API_END
+}
+
+int vox8_trial_mfma_apply(vox8_ctx* c, const float* d_X, float* d_Y, int nrhs) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_X && d_Y, "null buffer");
+  return launch_apply_mfma_trial(c->g, c->dKe32, c->fK32, c->d_fixed, d_X, d_Y, nrhs, c->st);
   API_END
 }
 
