@@ -254,6 +254,19 @@ def run_workload(name, precision, steps, warmup):
     return wl, geom, fem, opt, dt, pcg_it, lob_it
 
 
+def _release(*objs):
+    """Free the device state of finished optimizers (their contexts, streams
+    and worker threads) before the next workload runs."""
+    import gc
+    import torch
+    for o in objs:
+        close = getattr(o, "close", None)
+        if close is not None:
+            close()
+    gc.collect()
+    torch.cuda.synchronize()
+
+
 def bench_c4(precision, steps, warmup):
     """The 16 uneven designs: concurrent (one context, stream and thread per
     design) against one after another; design-iterations per second."""
@@ -271,6 +284,7 @@ def bench_c4(precision, steps, warmup):
         dt = time.perf_counter() - t0
         out[mode] = round(16 * steps / dt, 3)
         print(f"[bench] c4 {mode}: {16 * steps} design-iterations in {dt:.3f}s", file=sys.stderr, flush=True)
+        _release(*[d.opt for d in ds])
         del ds
     return out
 
@@ -315,6 +329,12 @@ def main():
                                                           args.warmup)
     value = args.steps / dt
     roofs = kernel_rooflines(fem, geom, opt.rho_phys)
+    # Each workload's contexts are released before the next one starts: every
+    # stream alive in the process shares the few hardware queues HIP maps
+    # streams onto, and a context kept alive for nothing can make the next
+    # workload's static and modal streams share a queue (and serialise).
+    _release(opt, fem)
+    del opt, fem
     plain = None
     if not args.no_plain and args.precision != "fp64":
         _, _, _, popt, _, _ = make_optimizer(args.workload, "fp64", modal_maxiter=args.plain_maxiter)
@@ -325,6 +345,7 @@ def main():
                  "pcg_iters": ph["pcg_iters"][-1], "lobpcg_iters": ph["lobpcg_iters"][-1],
                  "lobpcg_maxiter": args.plain_maxiter, "speedup_of_value": round(value * pdt, 2)}
         print(f"[bench] plain FP64 path: {pdt:.3f}s per SIMP iteration", file=sys.stderr, flush=True)
+        _release(popt)
         del popt
     secondary = {}
     also = [s for s in args.also.split(",") if s and s != args.workload]
@@ -337,6 +358,7 @@ def main():
                            "ms_per_step": round(sdt / args.steps * 1e3, 2), "n_dofs": sgeom.n_dofs,
                            "load_cases": len(swl.loads), "modes": swl.n_modes,
                            "mean_pcg_iters": round(spcg, 1), "mean_lobpcg_iters": round(slob, 1)}
+        _release(sopt)
         del sopt
     line = {
         "metric": "SIMP iters/s (solve+6 modes+adjoint) & matvec HBM GB/s vs plain FP64 path",

@@ -72,7 +72,8 @@ struct MultiGrid {
   uint64_t use_clock = 0;
   int capture_failures = 0;
   std::vector<MGGraph> graphs;
-  hipStream_t cap_st = nullptr;   // capture only (the null stream cannot be captured)
+  hipStream_t cap_st = nullptr;   // capture stream of a context on the null stream (which
+                                  // cannot be captured); created on first capture
   MultiGrid() = default;
   MultiGrid(const MultiGrid&) = delete;
   MultiGrid& operator=(const MultiGrid&) = delete;

@@ -709,28 +709,33 @@ MultiGrid::~MultiGrid() {
   if (cap_st) (void)hipStreamDestroy(cap_st);
 }
 
-// Capture the launches of one cycle on the capture stream and instantiate.
-// Captures are serialised process-wide (contexts on other threads may be
-// capturing too).  Returns false, with no error left pending, when the
-// runtime rejects or invalidates the capture; the caller then launches the
-// cycle directly.
+// Capture the launches of one cycle and instantiate.  The capture runs on the
+// context's stream, or, when that is the null stream (which cannot be
+// captured), on a private stream created on first use: every stream alive in
+// the process takes a share of the few hardware queues HIP maps streams onto
+// (GPU_MAX_HW_QUEUES), and once they are shared, work meant to overlap
+// serialises.  Captures are serialised process-wide (contexts on other
+// threads may be capturing too).  Returns false, with no error left pending,
+// when the runtime rejects or invalidates the capture; the caller then
+// launches the cycle directly.
 static std::mutex g_capture_mutex;
 static bool mg_capture(MultiGrid& mg, const double* r, double* z, int ncols, const int* skip,
-                       hipGraphExec_t* exec) {
+                       hipStream_t st, hipGraphExec_t* exec) {
   std::lock_guard<std::mutex> lock(g_capture_mutex);
   *exec = nullptr;
-  if (!mg.cap_st && hipStreamCreateWithFlags(&mg.cap_st, hipStreamNonBlocking) != hipSuccess) {
+  if (!st && !mg.cap_st && hipStreamCreateWithFlags(&mg.cap_st, hipStreamNonBlocking) != hipSuccess) {
     mg.cap_st = nullptr;
     (void)hipGetLastError();
     return false;
   }
-  if (hipStreamBeginCapture(mg.cap_st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+  hipStream_t cs = st ? st : mg.cap_st;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
-  const int rc = mg_apply_launches(mg, r, z, ncols, mg.cap_st, skip);
+  const int rc = mg_apply_launches(mg, r, z, ncols, cs, skip);
   hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(mg.cap_st, &graph);
+  const hipError_t ec = hipStreamEndCapture(cs, &graph);
   bool ok = rc == OK && ec == hipSuccess && graph != nullptr;
   if (ok) ok = hipGraphInstantiateWithFlags(exec, graph, 0) == hipSuccess;
   if (graph) (void)hipGraphDestroy(graph);
@@ -783,7 +788,7 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
     }
     MGGraph gr;
     gr.r = r; gr.z = z; gr.ncols = ncols; gr.skip = skip; gr.epoch = mg.epoch;
-    if (!mg_capture(mg, r, z, ncols, skip, &gr.exec)) {
+    if (!mg_capture(mg, r, z, ncols, skip, st, &gr.exec)) {
       // after repeated rejections stop trying; either way run this cycle directly
       if (++mg.capture_failures >= 4) mg.use_graphs = false;
       return mg_apply_launches(mg, r, z, ncols, st, skip);

@@ -289,6 +289,19 @@ class SIMPOptimizer:
         h["time_s"].append(time.perf_counter() - t0)
         return change
 
+    def close(self):
+        """Release the modal context, its worker thread and stream.  Streams
+        share a few hardware queues per process, so an optimiser kept alive
+        after its run can take the queues another one needs to overlap its
+        solves.  The optimiser keeps working afterwards, with the two solves
+        one after the other in the main context."""
+        pool = getattr(self, "_pool", None)
+        if pool is not None:
+            pool.shutdown(wait=True)
+        self._pool = None
+        self._fem_modal = None
+        self._modal_stream = None
+
     def _track(self, lam1, viol, it):
         """History and mu ramp of the frequency constraint (src/topopt.py:363-373)."""
         self.history["omega_1_Hz"].append(float(np.sqrt(lam1) / (2.0 * np.pi)))
