@@ -101,6 +101,13 @@ int vox8_pcg(vox8_ctx* ctx, const double* d_F, double* d_U, int nrhs, int64_t ld
 int vox8_lobpcg(vox8_ctx* ctx, int k, double tol, int maxiter, int prec, const double* d_X0,
                 double* d_X, double* h_lam, int* h_iters, int* h_info, double* h_max_res,
                 uint64_t seed);
+/* Guard columns of vox8_lobpcg (no reference counterpart; scipy's lobpcg has
+ * none): `guard` (0..2) extra columns are iterated with the k wanted ones but
+ * not returned or tested for convergence, so the last wanted pair converges
+ * at the rate set by lambda_{k+guard+1} instead of lambda_{k+1}; a warm start
+ * reuses the previous solve's guard columns.  -1 restores the default
+ * (environment VOX8_LOBPCG_GUARD, else 0). */
+int vox8_set_lobpcg_guard(vox8_ctx* ctx, int guard);
 
 /* The geometric-multigrid preconditioner used by the MIXED solvers:
  * z = cycle(r) (W-cycle by default) for nrhs FP64 column-major vectors (ld = n_dofs); r and z

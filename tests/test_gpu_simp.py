@@ -90,3 +90,26 @@ def test_device_tail_matches_host_tail(gpu, case):
             continue
         assert rel(y, x) < 1e-12, (k, x, y)
     assert rel(runs[True]["rho_active"], runs[False]["rho_active"]) < 1e-12
+
+
+def test_auto_guard_columns_keep_the_history(gpu):
+    """The "auto" guard policy (two guard columns once an iteration's modal
+    solve is slow; here forced from the second iteration on) changes how the
+    modes are reached, not what they are: the history equals the run without
+    guard columns to the golden test's bar (1e-6)."""
+    from vox8.fem import FEM3D, Material
+    from vox8.topopt import SIMPOptimizer
+
+    class Eager(SIMPOptimizer):
+        GUARD_TRIGGER = -1
+
+    geom, loads, opts, mat = tipbox_problem()
+    runs = {}
+    for cls, guard in ((SIMPOptimizer, 0), (Eager, "auto")):
+        fem = FEM3D(geom, Material(**mat), precision="mixed")
+        runs[guard] = cls(geom, fem, loads, solve_rtol=TIPBOX_SOLVE_RTOL, modal_tol=TIPBOX_MODAL_TOL,
+                          modal_maxiter=TIPBOX_MODAL_MAXITER, modal_guard=guard,
+                          **opts).run(verbose=False)
+    a, b = runs[0]["history"], runs["auto"]["history"]
+    for k in HIST:
+        assert rel(np.asarray(b[k], dtype=float), np.asarray(a[k], dtype=float)) < 1e-6, k

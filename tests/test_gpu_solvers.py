@@ -119,6 +119,39 @@ def test_lobpcg_matches_dense_eigh(gpu, k):
     assert np.abs(G - np.eye(k)).max() < 1e-8
 
 
+@pytest.mark.parametrize("guard,prec", [(1, "fp64"), (2, "fp64"), (2, "mixed")])
+def test_lobpcg_guard_columns_same_modes(gpu, guard, prec):
+    """Guard columns (vox8_set_lobpcg_guard) are iterated but not returned:
+    the k wanted pairs still match dense eigh, cold and warm started, and -1
+    restores the default."""
+    from vox8.fem import FEM3D, Material
+    geom = MaskedBox(8, 3, 3, hole_frac=0.1, seed=21)
+    ofem, _ = _pair(geom)
+    vfem = FEM3D(geom, Material(E=1.7e9, nu=0.4, rho=1010.0), precision=prec)
+    rho = random_density(ofem.n_active, 6, 0.3, 1.0)
+    free = ofem.free_dofs
+    Kff = ofem.assemble_K(rho)[free][:, free].toarray()
+    Mff = ofem.assemble_M(rho)[free][:, free].toarray()
+    k = 4
+    lam_ref, V = sla.eigh(Kff, Mff, subset_by_index=[0, k - 1])
+    tol = 1e-9 * 1.7e9 * geom.dx
+    vfem.set_lobpcg_guard(guard)
+    X0 = None
+    for _ in range(2):                      # cold, then warm (reusing the guard columns)
+        lam, X, info = vfem.modal_block(rho, k, 0.0, np.array([], dtype=np.int64), X0=X0, tol=tol,
+                                        maxiter=3000)
+        assert info["info"] == 0 and X.shape[0] == k
+        order = np.argsort(lam)
+        assert np.max(np.abs(lam[order] / lam_ref - 1)) < 1e-6
+        Xn = X.cpu().numpy()[order]
+        for j in range(k):
+            assert mac(Xn[j][free], V[:, j]) > 0.9999
+        X0 = X
+    vfem.set_lobpcg_guard(-1)
+    with pytest.raises(Exception):
+        vfem.set_lobpcg_guard(3)
+
+
 def test_lobpcg_tip_mass_and_warm_start(gpu):
     geom = MaskedBox(10, 3, 3, hole_frac=0.0, seed=2)
     ofem, vfem = _pair(geom)

@@ -69,6 +69,7 @@ struct vox8_ctx {
   // OC / sensitivities scratch
   double* d_oc = nullptr;
   double* d_tail = nullptr;         // TAIL_STATE doubles
+  int lobpcg_guard = -1;            // guard columns of vox8_lobpcg (-1: environment or 0)
   double *d_kphi = nullptr, *d_mphi = nullptr, *d_ce = nullptr, *d_sums = nullptr,
          *d_wts = nullptr;
 };
@@ -567,8 +568,11 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
   // optional guard columns (iterated, not returned; tuning switch
   // VOX8_LOBPCG_GUARD): fewer iterations when lambda_k+1 is close to lambda_k,
   // at a higher cost per iteration; a warm start reuses the last solve's
-  int guard = 0;
-  if (const char* e = getenv("VOX8_LOBPCG_GUARD")) guard = atoi(e);
+  int guard = c->lobpcg_guard;
+  if (guard < 0) {
+    guard = 0;
+    if (const char* e = getenv("VOX8_LOBPCG_GUARD")) guard = atoi(e);
+  }
   guard = std::max(0, std::min(std::min(guard, LOBPCG_GUARD_MAX), 12 - k));
   VOX8_TRY(ensure_lobpcg_work(c, k + guard));
   if (!d_X0 || c->lw.guard_k != k) c->lw.guard_cols = 0;   // cold start: random guard columns
@@ -595,6 +599,13 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
   if (h_max_res) *h_max_res = r.max_res;
   return OK;
   API_END
+}
+
+int vox8_set_lobpcg_guard(vox8_ctx* c, int guard) {
+  VOX8_CHECK(c != nullptr, "null context");
+  VOX8_CHECK(guard >= -1 && guard <= LOBPCG_GUARD_MAX, "guard must be -1 (environment) or 0..2");
+  c->lobpcg_guard = guard;
+  return OK;
 }
 
 int vox8_sensitivities(vox8_ctx* c, const double* d_U, int nrhs, int64_t ld, const double* h_w,

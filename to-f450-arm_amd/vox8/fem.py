@@ -359,6 +359,12 @@ class FEM3D:
                 setattr(t, k, getattr(self, k))
         return t
 
+    def set_lobpcg_guard(self, guard: int):
+        """Guard columns of the modal solve (0..2; -1 = the default): iterated
+        with the wanted modes, not returned; they speed up the last wanted mode
+        when the next eigenvalue is close to it."""
+        self._call("vox8_set_lobpcg_guard", int(guard))
+
     def modal_block(self, rho_active, n_modes=6, tip_mass_kg=0.060, tip_mass_nodes=None,
                     X0=None, tol=1e-4, maxiter=500, seed=0, precision=None):
         """Device-resident modal solve: returns (lam (k,) numpy, X (k, n_dofs) torch, info)."""

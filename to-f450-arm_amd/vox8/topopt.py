@@ -105,14 +105,21 @@ class SIMPOptimizer:
     (reference loop defaults 1e-4 / 500), solve_maxiter (5000), concurrent_solves
     (True: the modal solve overlaps the static one in a second context),
     device_tail (True: mode tracking, penalty, OC and bookkeeping in one device
-    call with one host read; False: the same steps driven from the host).
+    call with one host read; False: the same steps driven from the host),
+    modal_guard (guard columns of the modal solve: 0..2, or "auto": none until
+    an iteration's LOBPCG needs more than GUARD_TRIGGER iterations, then two;
+    late in a run the last tracked mode's neighbour comes close and the last
+    pair then converges slowly without them).
     """
+
+    GUARD_TRIGGER = 50
 
     def __init__(self, geom, fem: FEM3D, load_cases, vol_frac=0.30, r_min_mm=2.0, move=0.2,
                  max_iter=100, tol_change=0.01, weights=None, min_freq_Hz=None, tip_mass_kg=0.060,
                  n_modes_track=4, mu_freq_init=0.10, mu_freq_max=50.0, mu_freq_ramp_every=5,
                  mu_freq_ramp_factor=1.5, F_list=None, solve_rtol=None, modal_tol=1e-4,
-                 modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True, device_tail=True):
+                 modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True, device_tail=True,
+                 modal_guard="auto"):
         self.geom, self.fem, self.load_cases = geom, fem, load_cases
         self.vol_frac = vol_frac
         self.r_min_vox = max(1.0, r_min_mm / (geom.dx * 1e3))
@@ -127,6 +134,11 @@ class SIMPOptimizer:
         self.modal_tol, self.modal_maxiter = modal_tol, modal_maxiter
         self.concurrent_solves = bool(concurrent_solves)
         self.device_tail = bool(device_tail)
+        if modal_guard != "auto" and int(modal_guard) not in (0, 1, 2):
+            raise ValueError("modal_guard must be 'auto' or 0..2")
+        self.modal_guard = modal_guard
+        self._guard = 0 if modal_guard == "auto" else int(modal_guard)
+        self._guard_set = None
         self._fem_modal = None
         if F_list is None:
             F_list = []
@@ -224,6 +236,11 @@ class SIMPOptimizer:
         rho_phys = fem.active_from_grid(self._phys)
         self.rho_phys = rho_phys
         modal_job = None
+        if self.modal:
+            mfem = self._fem_modal if self._fem_modal is not None else fem
+            if self._guard_set != (id(mfem), self._guard):
+                mfem.set_lobpcg_guard(self._guard)
+                self._guard_set = (id(mfem), self._guard)
         if self.modal and self._fem_modal is not None:
             ready = torch.cuda.Event()
             ready.record()
@@ -267,6 +284,11 @@ class SIMPOptimizer:
                                                 self.tip, X0=self.X, tol=self.modal_tol,
                                                 maxiter=self.modal_maxiter)
             lob_iters = linfo["iters"]
+            if self.modal_guard == "auto" and lob_iters > self.GUARD_TRIGGER and self._guard != 2:
+                self._guard = 2
+                # the next modal solve is two columns wider: run it alone so
+                # the launch shapes of that width are tuned on an idle GPU
+                self._overlap = False
             order = np.argsort(lam)
             lam = np.maximum(lam[order], 0.0)
             if not np.array_equal(order, np.arange(len(order))):
