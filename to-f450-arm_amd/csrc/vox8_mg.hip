@@ -75,7 +75,17 @@ __global__ void k_mg_jacobi(Grid g, const double* __restrict__ f, Diag24 dg,
 }
 
 
-// coarse b = R r: weights 1 at the coincident fine node, 1/2 at the neighbours, per axis
+// A node's three components as one 12-byte access (global_load/store_dwordx3
+// on 4-byte alignment): a wave then touches each cache line of the
+// interleaved x, y, z layout once, where three strided scalar accesses each
+// touched all of them.
+struct F3 { float x, y, z; };
+__device__ __forceinline__ F3 ld_f3(const float* p) { return *reinterpret_cast<const F3*>(p); }
+__device__ __forceinline__ void st_f3(float* p, F3 v) { *reinterpret_cast<F3*>(p) = v; }
+
+// coarse b = R r: weights 1 at the coincident fine node, 1/2 at the neighbours, per axis.
+// Branch-free: a neighbour outside the fine grid is read at the clamped
+// position and added with weight 0 (same terms, same order as skipping it).
 __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
                               const uint8_t* __restrict__ fixc, float* __restrict__ bc,
                               const int* __restrict__ skip) {
@@ -86,29 +96,39 @@ __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
   rf += (int64_t)col * gf.n_dofs;
   bc += (int64_t)col * gc.n_dofs;
   const int I = (int)(n % gc.NX), J = (int)((n / gc.NX) % gc.NY), K = (int)(n / ((int64_t)gc.NX * gc.NY));
-  float s[3] = {0.f, 0.f, 0.f};
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
     const int z = 2 * K + dz;
-    if (z < 0 || z >= gf.NZ) continue;
-    const float wz = dz ? 0.5f : 1.0f;
+    const bool okz = z >= 0 && z < gf.NZ;
+    const int zc = min(max(z, 0), gf.NZ - 1);
+    const float wz = okz ? (dz ? 0.5f : 1.0f) : 0.0f;
+#pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
       const int y = 2 * J + dy;
-      if (y < 0 || y >= gf.NY) continue;
-      const float wy = wz * (dy ? 0.5f : 1.0f);
+      const bool oky = y >= 0 && y < gf.NY;
+      const int yc = min(max(y, 0), gf.NY - 1);
+      const float wy = oky ? wz * (dy ? 0.5f : 1.0f) : 0.0f;
+#pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const int x = 2 * I + dx;
-        if (x < 0 || x >= gf.NX) continue;
-        const float w = wy * (dx ? 0.5f : 1.0f);
-        const int64_t f = x + (int64_t)gf.NX * (y + (int64_t)gf.NY * z);
-        for (int c = 0; c < 3; ++c) s[c] += w * rf[3 * f + c];
+        const bool okx = x >= 0 && x < gf.NX;
+        const int xc = min(max(x, 0), gf.NX - 1);
+        const float w = okx ? wy * (dx ? 0.5f : 1.0f) : 0.0f;
+        const F3 v = ld_f3(rf + 3 * (xc + (int64_t)gf.NX * (yc + (int64_t)gf.NY * zc)));
+        s0 += w * v.x;
+        s1 += w * v.y;
+        s2 += w * v.z;
       }
     }
   }
   const uint8_t fx = fixc[n];
-  for (int c = 0; c < 3; ++c) bc[3 * n + c] = ((fx >> c) & 1) ? 0.0f : s[c];
+  st_f3(bc + 3 * n, F3{(fx & 1) ? 0.0f : s0, (fx & 2) ? 0.0f : s1, (fx & 4) ? 0.0f : s2});
 }
 
-// fine x += P coarse x (trilinear), fixed fine DOFs untouched
+// fine x += P coarse x (trilinear), fixed fine DOFs untouched.  Branch-free:
+// on an axis where the fine node coincides with a coarse one the second coarse
+// index equals the first and its term is not added.
 __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
                                  const uint8_t* __restrict__ fixf, float* __restrict__ xf,
                                  const int* __restrict__ skip) {
@@ -121,17 +141,28 @@ __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
   const int i = (int)(n % gf.NX), j = (int)((n / gf.NX) % gf.NY), k = (int)(n / ((int64_t)gf.NX * gf.NY));
   const int x0 = i >> 1, x1 = (i + 1) >> 1, y0 = j >> 1, y1 = (j + 1) >> 1, z0 = k >> 1, z1 = (k + 1) >> 1;
   const float wx = (x0 == x1) ? 1.0f : 0.5f, wy = (y0 == y1) ? 1.0f : 0.5f, wz = (z0 == z1) ? 1.0f : 0.5f;
-  float s[3] = {0.f, 0.f, 0.f};
-  for (int cz = 0; cz < (z0 == z1 ? 1 : 2); ++cz)
-    for (int cy = 0; cy < (y0 == y1 ? 1 : 2); ++cy)
-      for (int cx = 0; cx < (x0 == x1 ? 1 : 2); ++cx) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+  for (int cz = 0; cz < 2; ++cz)
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+      for (int cx = 0; cx < 2; ++cx) {
+        const bool use = (cz == 0 || z0 != z1) && (cy == 0 || y0 != y1) && (cx == 0 || x0 != x1);
         const int64_t c = (cx ? x1 : x0) + (int64_t)gc.NX * ((cy ? y1 : y0) + (int64_t)gc.NY * (cz ? z1 : z0));
-        for (int q = 0; q < 3; ++q) s[q] += xc[3 * c + q];
+        const F3 v = ld_f3(xc + 3 * c);
+        s0 += use ? v.x : 0.0f;
+        s1 += use ? v.y : 0.0f;
+        s2 += use ? v.z : 0.0f;
       }
   const float w = wx * wy * wz;
   const uint8_t fx = fixf[n];
-  for (int q = 0; q < 3; ++q)
-    if (!((fx >> q) & 1)) xf[3 * n + q] += w * s[q];
+  float* p = xf + 3 * n;
+  F3 o = ld_f3(p);
+  o.x = (fx & 1) ? o.x : o.x + w * s0;
+  o.y = (fx & 2) ? o.y : o.y + w * s1;
+  o.z = (fx & 4) ? o.z : o.z + w * s2;
+  st_f3(p, o);
 }
 
 // dense coarse matrix A[i][j] (identity rows/cols at fixed DOFs)
