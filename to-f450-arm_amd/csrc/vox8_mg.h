@@ -50,7 +50,7 @@ struct MultiGrid {
   int ncols = 0;               // capacity of the work vectors
   int nu = 2;                  // Chebyshev smoothing steps before and after
   int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
-  int64_t w_min_dofs = 50000;  // levels smaller than this take one correction (V) even when
+  int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when
                                // gamma = 2: their kernels are launch-latency bound
   int coarse_dofs = 700;       // stop coarsening at this many DOFs (dense inverse)
   double alpha = 12.0;         // smoothing interval [lam_max/alpha, lam_max]
