@@ -1,6 +1,6 @@
 """GPU parity: adjoint sensitivities, cone filter and OC update vs the oracle.
 
-Sensitivity bar 1e-5 relative (north star); the kernel computes the same
+Sensitivity bar 1e-5 relative (the parity target); the kernel computes the same
 closed forms, so agreement is checked at 1e-12.  The FD check perturbs the
 physical density of sampled elements and differences stationary functionals
 (compliance 2F.u - u.K.u, Rayleigh quotient), as the adjoint formulas are the

@@ -3,7 +3,7 @@ configuration (c4).
 
 c3: 220x32x24 at 1 mm, LC1-LC5 solved together as one block PCG (rtol 1e-8)
 against each load case solved alone: per-load-case compliance within 1e-6
-displacements within 1e-6, and the weighted summed
+(the parity bar), displacements within 1e-6, and the weighted summed
 compliance gradient equal to the weighted sum of the single-case gradients.
 
 c4: the 16 uneven designs (nx 80..220, V_f 0.08..0.50), two SIMP iterations
