@@ -15,6 +15,8 @@ Prints one JSON line.  Extra objects:
                 launch in "alg_bytes_per_launch"; "traffic" from the committed
                 rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
   kernels       the same measurement for the other hot kernels
+  solvers       the static (PCG) and modal (LOBPCG) solves alone on the final
+                design, cold-started: wall time, iterations, PCG iterations/s
   plain_fp64    one SIMP iteration of the same loop on the plain FP64 path
                 (the reference's Jacobi preconditioner in PCG and LOBPCG)
   secondary     the other workloads given with --also (default c2, c3 and c4,
@@ -254,6 +256,41 @@ def run_workload(name, precision, steps, warmup):
     return wl, geom, fem, opt, dt, pcg_it, lob_it
 
 
+def solver_timings(opt):
+    """The static and modal solves alone (no overlap), cold-started (U = 0, a
+    random LOBPCG block) on the optimiser's final design: wall time,
+    iterations, and PCG iterations/s (one block iteration advances every load
+    case, so column iterations/s = iterations/s x load cases).  Inside the
+    SIMP loop both solves are warm-started from the previous iteration's
+    solutions; their mean iteration counts are in the config."""
+    import torch
+    fem, rho = opt.fem, opt.rho_phys
+    mfem = opt._fem_modal if opt._fem_modal is not None else fem
+    mst = opt._modal_stream if opt._fem_modal is not None else torch.cuda.current_stream()
+
+    def clock(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    dt, (_, info) = clock(lambda: fem.solve_block(rho, opt.F, rtol=opt.solve_rtol,
+                                                  maxiter=opt.solve_maxiter))
+    it, nlc = max(info["iters"]), opt.F.shape[0]
+    out = {"pcg": {"s": round(dt, 4), "iters": it, "iters_per_s": round(it / dt, 1),
+                   "column_iters_per_s": round(it * nlc / dt, 1), "load_cases": nlc,
+                   "rtol": opt.solve_rtol, "start": "cold"}}
+    if opt.modal:
+        with torch.cuda.stream(mst):
+            dt, (_, _, info) = clock(lambda: mfem.modal_block(
+                rho, opt.n_modes_track, opt.tip_mass_kg, opt.tip, tol=opt.modal_tol,
+                maxiter=opt.modal_maxiter))
+        out["lobpcg"] = {"s": round(dt, 4), "iters": info["iters"], "modes": opt.n_modes_track,
+                         "tol": opt.modal_tol, "start": "cold"}
+    return out
+
+
 def _release(*objs):
     """Free the device state of finished optimizers (their contexts, streams
     and worker threads) before the next workload runs."""
@@ -328,6 +365,8 @@ def main():
     wl, geom, fem, opt, dt, pcg_it, lob_it = run_workload(args.workload, args.precision, args.steps,
                                                           args.warmup)
     value = args.steps / dt
+    solvers = solver_timings(opt)
+    print(f"[bench] solves alone: {json.dumps(solvers)}", file=sys.stderr, flush=True)
     roofs = kernel_rooflines(fem, geom, opt.rho_phys)
     # Each workload's contexts are released before the next one starts: every
     # stream alive in the process shares the few hardware queues HIP maps
@@ -349,9 +388,9 @@ def main():
         del popt
     secondary = {}
     also = [s for s in args.also.split(",") if s and s != args.workload]
-    if "c4" in also:
+    run_c4 = "c4" in also
+    if run_c4:
         also.remove("c4")
-        secondary["c4"] = bench_c4(args.precision, args.steps, args.warmup)
     for name in also:
         swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, args.steps, args.warmup)
         secondary[name] = {"value": round(args.steps / sdt, 4), "unit": "SIMP iters/s",
@@ -360,6 +399,11 @@ def main():
                            "mean_pcg_iters": round(spcg, 1), "mean_lobpcg_iters": round(slob, 1)}
         _release(sopt)
         del sopt
+    if run_c4:
+        # last: its 16 per-design streams come from torch's stream pool and
+        # stay alive, so a workload after it could get a modal stream on the
+        # same hardware queue as its static stream (and the two solves serialise)
+        secondary["c4"] = bench_c4(args.precision, args.steps, args.warmup)
     line = {
         "metric": "SIMP iters/s (solve+6 modes+adjoint) & matvec HBM GB/s vs plain FP64 path",
         "value": round(value, 4), "unit": "SIMP iters/s", "n_gpus": 1, "steps": args.steps,
@@ -374,6 +418,7 @@ def main():
                    "precision": args.precision},
         "roofline": roofs[DOMINANT],
         "kernels": {k: v for k, v in roofs.items() if k != DOMINANT},
+        "solvers": solvers,
         "plain_fp64": plain,
         "secondary": secondary,
     }
