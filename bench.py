@@ -291,6 +291,27 @@ def solver_timings(opt):
     return out
 
 
+def trajectory(name, precision, n_iter, modal_maxiter):
+    """n_iter SIMP iterations from the seeded density, each timed on its own
+    (the first includes the launch-shape tuning of the multigrid path)."""
+    import torch
+    wl, geom, fem, opt, F, x0 = make_optimizer(name, precision, modal_maxiter=modal_maxiter)
+    secs = []
+    for i in range(n_iter):
+        secs.append(time_steps(opt, 0, 1))
+        h = opt.history
+        print(f"[bench] {name} {precision} iteration {i + 1}: {secs[-1]:.3f}s, pcg {h['pcg_iters'][-1]}, "
+              f"lobpcg {h['lobpcg_iters'][-1]}, f1 {h['omega_1_Hz'][-1]:.6f} Hz", file=sys.stderr, flush=True)
+    h = opt.history
+    out = {"workload": name, "precision": precision, "seconds": [round(t, 4) for t in secs],
+           "pcg_iters": h["pcg_iters"], "lobpcg_iters": h["lobpcg_iters"],
+           "lobpcg_maxiter": modal_maxiter, "compliance": h["compliance"], "omega_1_Hz": h["omega_1_Hz"]}
+    _release(opt, fem)
+    del opt, fem
+    torch.cuda.synchronize()
+    return out
+
+
 def _release(*objs):
     """Free the device state of finished optimizers (their contexts, streams
     and worker threads) before the next workload runs."""
@@ -339,6 +360,10 @@ def main():
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--trajectory", type=int, default=0, metavar="N",
+                    help="instead of the bench line: N SIMP iterations of the workload on the "
+                         "multigrid path and on the plain FP64 path from the same seeded "
+                         "density, one JSON line per path with per-iteration times")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-kernel timings (and a 1 GiB device copy as a counter "
                          "calibration), for rocprofv3 --pmc runs")
@@ -349,6 +374,12 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+
+    if args.trajectory > 0:
+        for prec in ("mixed", "fp64"):
+            print(json.dumps(trajectory(args.workload, prec, args.trajectory, args.plain_maxiter),
+                             default=lambda v: v.item()), flush=True)
+        return
 
     if args.kernels_only:
         wl, geom, fem, opt, F, x0 = make_optimizer(args.workload, args.precision)
