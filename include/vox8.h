@@ -130,7 +130,7 @@ int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, 
  * [lam_max/alpha, lam_max], coarsest-level size (DOFs) for the dense inverse,
  * cycle index (1 V-cycle, 2 W-cycle) applied on levels with at least
  * w_min_dofs DOFs (smaller levels take one coarse correction).  Defaults:
- * nu 2, alpha 12, coarse_dofs 700, cycle 2, w_min_dofs 5000. */
+ * nu 2, alpha 20, coarse_dofs 700, cycle 2, w_min_dofs 5000. */
 int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, int cycle,
                         int64_t w_min_dofs);
 /* Centre-point element stresses, Voigt order (xx, yy, zz, xy, yz, xz), linear

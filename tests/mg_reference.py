@@ -53,7 +53,7 @@ def prolongation(g_f, g_c, free_f, free_c):
 
 
 class VCycle:
-    def __init__(self, dims, f, Ke, fixed_nodes_mask, active=None, nu=2, alpha=12.0, coarse_dofs=700,
+    def __init__(self, dims, f, Ke, fixed_nodes_mask, active=None, nu=2, alpha=20.0, coarse_dofs=700,
                  gamma=2, w_min_dofs=0):
         self.nu, self.alpha, self.gamma, self.w_min_dofs = nu, alpha, gamma, w_min_dofs
         nx, ny, nz = dims

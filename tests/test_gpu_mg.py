@@ -33,7 +33,7 @@ def _setup(shape, seed, hole):
 def test_vcycle_matches_reference(gpu, shape, hole, gamma, wmin):
     from mg_reference import VCycle
     geom, fem, rho, _lib = _setup(shape, 3, hole)
-    fem._call("vox8_set_mg_options", 2, 12.0, 700, gamma, wmin)
+    fem._call("vox8_set_mg_options", 2, 20.0, 700, gamma, wmin)
     nl = C.c_int()
     dims = (C.c_int * 48)()
     lam = (C.c_double * 16)()
@@ -41,7 +41,7 @@ def test_vcycle_matches_reference(gpu, shape, hole, gamma, wmin):
     f = np.zeros(geom.n_elements)
     f[fem.active_elems] = fem.E_MIN_FRAC + (1 - fem.E_MIN_FRAC) * rho ** 3
     V = VCycle((geom.nx, geom.ny, geom.nz), f, fem.Ke_solid, fem.node_fixed,
-               active=fem.elem_active_grid, gamma=gamma, w_min_dofs=wmin)
+               active=fem.elem_active_grid, alpha=20.0, gamma=gamma, w_min_dofs=wmin)
     assert [L["dims"] for L in V.levels] == [tuple(dims[3 * i:3 * i + 3]) for i in range(nl.value)]
     for l, L in enumerate(V.levels[:-1]):
         L["lam_max"] = lam[l]

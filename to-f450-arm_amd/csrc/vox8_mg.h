@@ -53,7 +53,7 @@ struct MultiGrid {
   int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when
                                // gamma = 2: their kernels are launch-latency bound
   int coarse_dofs = 700;       // stop coarsening at this many DOFs (dense inverse)
-  double alpha = 12.0;         // smoothing interval [lam_max/alpha, lam_max]
+  double alpha = 20.0;         // smoothing interval [lam_max/alpha, lam_max]
   // coarsest level: dense inverse when small
   int n_coarse = 0;
   double* A = nullptr;         // n_coarse^2 FP64 (assembled, then inverted in place)
