@@ -20,7 +20,11 @@ Prints one JSON line.  Extra objects:
   plain_fp64    one SIMP iteration of the same loop on the plain FP64 path
                 (the reference's Jacobi preconditioner in PCG and LOBPCG)
   secondary     the other workloads given with --also (default c2, c3 and c4,
-                the 16 batched designs run concurrently and one by one)
+                the 16 batched designs run concurrently and one by one, in a
+                child process)
+
+Other modes: --trajectory N (per-iteration times of both paths over N SIMP
+iterations), --kernels-only (kernel timings for counter runs), --c4-only.
   cpu_baseline  the SciPy oracle on this host (bounded z-slab sample of the
                 same grid, scaled by DOF count and by the plain path's
                 iteration counts, i.e. the reference's algorithm)
