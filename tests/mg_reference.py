@@ -54,15 +54,18 @@ def prolongation(g_f, g_c, free_f, free_c):
 
 class VCycle:
     def __init__(self, dims, f, Ke, fixed_nodes_mask, active=None, nu=2, alpha=20.0, coarse_dofs=700,
-                 gamma=2, w_min_dofs=0):
+                 gamma=2, w_min_dofs=0, galerkin=False):
         self.nu, self.alpha, self.gamma, self.w_min_dofs = nu, alpha, gamma, w_min_dofs
         nx, ny, nz = dims
         fixed = fixed_nodes_mask.astype(np.uint8) * 7 if fixed_nodes_mask.dtype == bool else fixed_nodes_mask
         act = np.ones(nx * ny * nz, bool) if active is None else active.astype(bool)
         self.levels = []
         scale = 1.0
+        Kg = None
         while True:
             K, free = assemble(nx, ny, nz, f, scale * Ke, fixed)
+            if Kg is not None:
+                K = Kg
             d = K.diagonal()
             dinv = np.where(free, 1.0 / np.where(d != 0, d, 1.0), 0.0)
             self.levels.append(dict(dims=(nx, ny, nz), K=K, free=free, dinv=dinv))
@@ -102,6 +105,9 @@ class VCycle:
             fixed_c = fc_fix.ravel()
             _, free_c = assemble(cx, cy, cz, fc, Ke, fixed_c)
             self.levels[-1]["P"] = prolongation((nx, ny, nz), (cx, cy, cz), free, free_c)
+            if galerkin:
+                Pl = self.levels[-1]["P"]
+                Kg = (Pl.T @ self.levels[-1]["K"] @ Pl).tocsr()
             nx, ny, nz, f, fixed, act = cx, cy, cz, fc, fixed_c, ac
             scale *= 2.0
         for L in self.levels[:-1]:

@@ -17,6 +17,10 @@
 namespace vox8 {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
+std::recursive_mutex& hip_serial() {
+  static std::recursive_mutex m;
+  return m;
+}
 }  // namespace vox8
 
 using namespace vox8;
@@ -78,6 +82,7 @@ namespace {
 
 template <typename T>
 int dalloc(vox8_ctx* c, T** p, size_t count) {
+  HipSerial lock(hip_serial());
   void* q = nullptr;
   VOX8_HIP(hipMalloc(&q, count * sizeof(T) + 256));
   c->dev_allocs.push_back(q);
@@ -86,6 +91,7 @@ int dalloc(vox8_ctx* c, T** p, size_t count) {
 }
 template <typename T>
 int halloc(vox8_ctx* c, T** p, size_t count) {
+  HipSerial lock(hip_serial());
   void* q = nullptr;
   VOX8_HIP(hipHostMalloc(&q, count * sizeof(T) + 64, hipHostMallocDefault));
   c->host_allocs.push_back(q);
@@ -94,11 +100,13 @@ int halloc(vox8_ctx* c, T** p, size_t count) {
 }
 void dfree(vox8_ctx* c, void* p) {
   if (!p) return;
+  HipSerial lock(hip_serial());
   for (auto it = c->dev_allocs.begin(); it != c->dev_allocs.end(); ++it)
     if (*it == p) { (void)hipFree(p); c->dev_allocs.erase(it); return; }
 }
 void hfree(vox8_ctx* c, void* p) {
   if (!p) return;
+  HipSerial lock(hip_serial());
   for (auto it = c->host_allocs.begin(); it != c->host_allocs.end(); ++it)
     if (*it == p) { (void)hipHostFree(p); c->host_allocs.erase(it); return; }
 }
@@ -276,6 +284,7 @@ int vox8_device_count(int* h_count) {
 
 int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   API_BEGIN
+  HipSerial lock(hip_serial());
   VOX8_CHECK(out != nullptr, "null output");
   VOX8_CHECK(nx > 0 && ny > 0 && nz > 0, "grid dimensions must be positive");
   VOX8_CHECK((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "grid too large");
@@ -315,6 +324,7 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
 
 int vox8_destroy(vox8_ctx* c) {
   if (!c) return OK;
+  HipSerial lock(hip_serial());
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st); else (void)hipDeviceSynchronize();
   for (void* p : c->dev_allocs) (void)hipFree(p);

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,16 @@ enum Status : int {
 };
 
 void set_error(const std::string& msg);
+
+// Process-wide lock held around every HIP call that allocates, frees,
+// creates or destroys a stream, event or graph, or waits on the host for the
+// device (device, stream and event synchronisation), and around each stream
+// capture.  Contexts may be driven from several host threads at once (one per
+// design or per solve); the lock keeps those calls from overlapping another
+// thread's capture, which the runtime does not allow.  Recursive: a locked
+// section may call helpers that lock again.
+std::recursive_mutex& hip_serial();
+using HipSerial = std::lock_guard<std::recursive_mutex>;
 
 #define VOX8_HIP(call)                                                        \
   do {                                                                        \

@@ -21,7 +21,6 @@
 #include <algorithm>
 #include <cstdlib>
 
-#include <mutex>
 
 #include "vox8_mg.h"
 
@@ -730,12 +729,14 @@ static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncol
 }
 
 void mg_drop_graphs(MultiGrid& mg) {
+  HipSerial lock(hip_serial());
   for (auto& gr : mg.graphs)
     if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
   mg.graphs.clear();
 }
 
 MultiGrid::~MultiGrid() {
+  HipSerial lock(hip_serial());
   mg_drop_graphs(*this);
   if (cap_st) (void)hipStreamDestroy(cap_st);
 }
@@ -745,14 +746,13 @@ MultiGrid::~MultiGrid() {
 // captured), on a private stream created on first use: every stream alive in
 // the process takes a share of the few hardware queues HIP maps streams onto
 // (GPU_MAX_HW_QUEUES), and once they are shared, work meant to overlap
-// serialises.  Captures are serialised process-wide (contexts on other
-// threads may be capturing too).  Returns false, with no error left pending,
-// when the runtime rejects or invalidates the capture; the caller then
-// launches the cycle directly.
-static std::mutex g_capture_mutex;
+// serialises.  A capture holds the process-wide HIP lock, so no other
+// thread allocates, frees or synchronises while it is open.  Returns false,
+// with no error left pending, when the runtime rejects or invalidates the
+// capture; the caller then launches the cycle directly.
 static bool mg_capture(MultiGrid& mg, const double* r, double* z, int ncols, const int* skip,
                        hipStream_t st, hipGraphExec_t* exec) {
-  std::lock_guard<std::mutex> lock(g_capture_mutex);
+  HipSerial lock(hip_serial());
   *exec = nullptr;
   if (!st && !mg.cap_st && hipStreamCreateWithFlags(&mg.cap_st, hipStreamNonBlocking) != hipSuccess) {
     mg.cap_st = nullptr;
@@ -788,6 +788,7 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
     // drop stale entries; keep at most 8 live ones (least recently used out)
     for (size_t i = 0; i < mg.graphs.size();) {
       if (mg.graphs[i].epoch != mg.epoch) {
+        HipSerial lock(hip_serial());
         (void)hipGraphExecDestroy(mg.graphs[i].exec);
         mg.graphs.erase(mg.graphs.begin() + i);
       } else {
@@ -798,7 +799,10 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
       size_t old = 0;
       for (size_t i = 1; i < mg.graphs.size(); ++i)
         if (mg.graphs[i].last_use < mg.graphs[old].last_use) old = i;
-      (void)hipGraphExecDestroy(mg.graphs[old].exec);
+      {
+        HipSerial lock(hip_serial());
+        (void)hipGraphExecDestroy(mg.graphs[old].exec);
+      }
       mg.graphs.erase(mg.graphs.begin() + old);
     }
     // FP32 stencil shapes of every smoothed level for this width, measured

@@ -701,6 +701,7 @@ int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, cons
                       const uint8_t* fixed, const float* dinv, float* d, float* r, float* xs,
                       float* dnew, int64_t ld, int ncols, hipStream_t st) {
   if (stencil_tuned_fp32(g, ncols)) return OK;
+  HipSerial hip_lock(hip_serial());     // event creation and host waits
   Epilogue<float> ep;
   ep.mode = EPI_CHEB;
   ep.r = r;

@@ -6,6 +6,7 @@ if it is missing or no GPU is visible, every GPU entry point raises.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libvox8.so"
@@ -107,6 +108,44 @@ def call(name, *args):
         msg = lib.vox8_last_error().decode(errors="replace")
         raise Vox8Error(f"{name} failed (status {st}): {msg}")
     return st
+
+
+_pending = []
+_pending_lock = threading.Lock()
+
+
+def _drain():
+    with _pending_lock:
+        ctxs = list(_pending)
+        _pending.clear()
+    lib = load()
+    for c in ctxs:
+        lib.vox8_destroy(c)
+
+
+def create(nx, ny, nz, stream):
+    """New context for an nx x ny x nz grid bound to `stream` (a HIP stream
+    handle as int, 0 = default stream).  Contexts released by the garbage
+    collector since the last call are destroyed first, on this thread."""
+    _drain()
+    ctx = C.c_void_p()
+    call("vox8_create", C.byref(ctx), int(nx), int(ny), int(nz), C.c_void_p(stream))
+    return ctx
+
+
+def destroy(ctx):
+    """Destroy a context now (and any released ones)."""
+    _drain()
+    if ctx is not None and ctx.value:
+        load().vox8_destroy(ctx)
+
+
+def release(ctx):
+    """Hand a context over for destruction at the next create()/destroy():
+    what a finaliser calls, so no library call is made from the collector."""
+    if ctx is not None and ctx.value:
+        with _pending_lock:
+            _pending.append(C.c_void_p(ctx.value))
 
 
 def ptr(t):

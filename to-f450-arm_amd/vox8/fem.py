@@ -2,9 +2,11 @@
 
 Drop-in differences, all deliberate:
 
-* ``assemble_K`` / ``assemble_M`` return a matrix-free ``VoxelOperator``
-  (no COO/CSR is ever built); ``apply_BCs_and_solve`` accepts that operator.
-  Passing an assembled scipy matrix raises TypeError.
+* ``assemble_K`` / ``assemble_M`` return a matrix-free ``VoxelOperator``;
+  ``apply_BCs_and_solve`` accepts that operator (an assembled scipy matrix
+  raises TypeError).  The solvers never assemble.  Callers that index, slice
+  or count the matrix (``M[fd][:, fd]``, ``K.nnz``, ``tocsr()``) get the
+  reference's CSR, assembled on the host on first such use.
 * Every product, solve and sensitivity runs in libvox8.so on the GPU; there
   is no CPU fallback.  Non-convergence is reported as a RuntimeWarning and the
   best iterate is returned (the reference silently switches to spsolve /
@@ -20,6 +22,7 @@ import warnings
 from dataclasses import dataclass
 
 import numpy as np
+import scipy.sparse as sp
 import torch
 
 from . import _lib
@@ -72,7 +75,12 @@ class VoxelOperator:
     """Matrix-free K(rho) or M(rho) on the full DOF vector (what assemble_* return).
 
     ``A @ x`` applies the free-DOF operator (x read as zero at fixed DOFs, zero
-    rows there), i.e. the embedding of K_ff / M_ff used by the solvers.
+    rows there), i.e. the embedding of K_ff / M_ff used by the solvers; it
+    equals the assembled product on every vector that is zero at the fixed
+    DOFs.  Indexing, ``nnz``, ``tocsr()`` and ``toarray()`` use the assembled
+    matrix of src/fem.py:311-325 / 362-398 (all DOFs, M with the tip lump),
+    built on the host once per operator for callers off the solver path such
+    as sims/modal_participation.py:44-49.
     """
 
     def __init__(self, fem, kind, rho, tip_mass_kg=0.0, tip_nodes=None):
@@ -81,6 +89,23 @@ class VoxelOperator:
         self.tip_nodes = np.array([], dtype=np.int64) if tip_nodes is None else np.asarray(tip_nodes, dtype=np.int64)
         n = fem.geom.n_dofs
         self.shape = (n, n)
+        self.dtype = np.dtype(np.float64)
+        self._csr = None
+
+    def tocsr(self):
+        if self._csr is None:
+            self._csr = self.fem._host_csr(self.kind, self.rho, self.tip_mass_kg, self.tip_nodes)
+        return self._csr
+
+    def toarray(self):
+        return self.tocsr().toarray()
+
+    @property
+    def nnz(self):
+        return self.tocsr().nnz
+
+    def __getitem__(self, key):
+        return self.tocsr()[key]
 
     def _prepare(self):
         self.fem._set_density(self.rho)
@@ -158,9 +183,8 @@ class FEM3D:
         self._elem_node_ids = None
         # device context
         self.device = _device()
-        self._ctx = C.c_void_p()
         self._stream = torch.cuda.current_stream(self.device).cuda_stream
-        call("vox8_create", C.byref(self._ctx), nx, ny, nz, C.c_void_p(self._stream))
+        self._ctx = _lib.create(nx, ny, nz, self._stream)
         self._Ke_dev = None
         self._Me_dev = None
         self._sync_element_matrices()
@@ -174,16 +198,26 @@ class FEM3D:
         self.last_info = {}
 
     # ------------------------------------------------------------------ plumbing
+    def close(self):
+        """Release the device context (its work space and captured graphs).
+        The object is unusable afterwards.  Contexts still open when they are
+        garbage-collected are handed to ``_lib.release`` instead, which
+        destroys them on the next ``close()`` / context creation of the
+        process, never from the collector's thread."""
+        ctx = getattr(self, "_ctx", None)
+        self._ctx = None
+        if ctx is not None and ctx.value:
+            _lib.destroy(ctx)
+
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
         if ctx is not None and ctx.value:
-            try:
-                _lib.load().vox8_destroy(ctx)
-            except Exception:
-                pass
+            _lib.release(ctx)
             self._ctx = None
 
     def _call(self, name, *args):
+        if self._ctx is None:
+            raise _lib.Vox8Error("FEM3D context is closed")
         s = torch.cuda.current_stream(self.device).cuda_stream
         if s != self._stream:
             self._stream = s
@@ -293,6 +327,32 @@ class FEM3D:
     def assemble_M(self, rho_active, tip_mass_kg: float = 0.0, tip_mass_nodes=None):
         return VoxelOperator(self, "M", rho_active, tip_mass_kg,
                              tip_mass_nodes if tip_mass_nodes is not None else [])
+
+    def _host_csr(self, kind, rho_active, tip_mass_kg=0.0, tip_nodes=()):
+        """Assembled K(rho) or M(rho) over all DOFs as scipy CSR (duplicates
+        summed, src/fem.py:311-325, 362-381) with the tip lump of
+        src/fem.py:383-398 on M.  Host-side, for off-solver callers only."""
+        rho = (rho_active.detach().cpu().numpy() if isinstance(rho_active, torch.Tensor)
+               else np.asarray(rho_active, dtype=np.float64))
+        if kind == "K":
+            f = self.E_MIN_FRAC + (1.0 - self.E_MIN_FRAC) * rho ** self.SIMP_PENALTY
+            Ae = np.asarray(self.Ke_solid, dtype=np.float64)
+        else:
+            f = self._mass_simp_factor(rho)
+            Ae = np.asarray(self.Me_solid, dtype=np.float64)
+        d = self.elem_dof_ids
+        n = self.geom.n_dofs
+        rows = np.repeat(d, 24, axis=1).ravel()
+        cols = np.tile(d, (1, 24)).ravel()
+        vals = (f[:, None] * Ae.ravel()[None, :]).ravel()
+        A = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+        nodes = np.asarray(tip_nodes, dtype=np.int64)
+        if kind == "M" and tip_mass_kg > 0.0 and len(nodes):
+            dofs = np.concatenate([3 * nodes, 3 * nodes + 1, 3 * nodes + 2])
+            lump = sp.coo_matrix((np.full(len(dofs), tip_mass_kg / len(nodes)), (dofs, dofs)),
+                                 shape=(n, n)).tocsr()
+            A = (A + lump).tocsr()
+        return A
 
     def _mass_simp_factor(self, rho_active):
         t, q = self.MASS_THRESHOLD, self.MASS_PENALTY_Q

@@ -55,16 +55,16 @@ class _GridContext:
 
     def __init__(self, nx, ny, nz):
         self.device = torch.device("cuda", torch.cuda.current_device())
-        self.ctx = C.c_void_p()
-        call("vox8_create", C.byref(self.ctx), nx, ny, nz,
-             C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        self.ctx = _lib.create(nx, ny, nz, torch.cuda.current_stream().cuda_stream)
+
+    def close(self):
+        ctx, self.ctx = getattr(self, "ctx", None), None
+        _lib.destroy(ctx)
 
     def __del__(self):
-        if getattr(self, "ctx", None) is not None and self.ctx.value:
-            try:
-                _lib.load().vox8_destroy(self.ctx)
-            except Exception:
-                pass
+        if getattr(self, "ctx", None) is not None:
+            _lib.release(self.ctx)
+            self.ctx = None
 
     def _call(self, name, *args):
         call("vox8_set_stream", self.ctx, C.c_void_p(torch.cuda.current_stream().cuda_stream))
@@ -311,18 +311,26 @@ class SIMPOptimizer:
         h["time_s"].append(time.perf_counter() - t0)
         return change
 
-    def close(self):
-        """Release the modal context, its worker thread and stream.  Streams
+    def close(self, fem: bool = False):
+        """Release the modal context, its worker thread and stream, in that
+        order (the thread is joined before its context is destroyed).  Streams
         share a few hardware queues per process, so an optimiser kept alive
         after its run can take the queues another one needs to overlap its
         solves.  The optimiser keeps working afterwards, with the two solves
-        one after the other in the main context."""
+        one after the other in the main context; ``fem=True`` also closes the
+        main FEM3D context (the optimiser is then unusable)."""
         pool = getattr(self, "_pool", None)
         if pool is not None:
             pool.shutdown(wait=True)
         self._pool = None
+        mf = getattr(self, "_fem_modal", None)
+        if mf is not None:
+            torch.cuda.synchronize(mf.device)
+            mf.close()
         self._fem_modal = None
         self._modal_stream = None
+        if fem:
+            self.fem.close()
 
     def _track(self, lam1, viol, it):
         """History and mu ramp of the frequency constraint (src/topopt.py:363-373)."""
