@@ -329,13 +329,18 @@ def _release(*objs):
     torch.cuda.synchronize()
 
 
+C4_MAX_STEPS, C4_MAX_WARMUP = 5, 1
+
+
 def bench_c4(precision, steps, warmup):
     """The 16 uneven designs: concurrent (one context, stream and thread per
-    design) against one after another; design-iterations per second."""
-    from vox8.batch import make_designs, step_designs
+    design) against one after another; design-iterations per second.  Each
+    phase closes its optimisers (worker threads joined, then contexts
+    destroyed) before the next starts."""
+    from vox8.batch import close_designs, make_designs, step_designs
     from vox8.workloads import c4_designs
     import torch
-    out = {"unit": "design SIMP iters/s", "designs": 16}
+    out = {"unit": "design SIMP iters/s", "designs": 16, "steps": steps, "warmup": warmup}
     for mode, conc in (("concurrent", True), ("sequential", False)):
         ds = make_designs(c4_designs(), precision, concurrent=conc)
         step_designs(ds, warmup, concurrent=conc)
@@ -345,29 +350,39 @@ def bench_c4(precision, steps, warmup):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         out[mode] = round(16 * steps / dt, 3)
+        out[mode + "_mean_pcg_iters"] = round(float(sum(sum(d.opt.history["pcg_iters"][-1])
+                                                        for d in ds)) / 16, 1)
+        out[mode + "_mean_lobpcg_iters"] = round(float(sum(d.opt.history["lobpcg_iters"][-1]
+                                                           for d in ds)) / 16, 1)
         print(f"[bench] c4 {mode}: {16 * steps} design-iterations in {dt:.3f}s", file=sys.stderr, flush=True)
-        _release(*[d.opt for d in ds])
+        close_designs(ds)
         del ds
     return out
 
 
-def bench_c4_child(precision, steps, warmup, timeout_s=300):
-    """bench_c4 in a child process with a fresh stream pool.  The concurrent
-    phase has ended in a host segmentation fault when it ran in this process
-    after the other workloads (cause not found); as a child, a failure is
-    reported in the bench line instead of ending the bench."""
+def bench_c4_child(precision, steps, warmup):
+    """bench_c4 in a child process (a fresh stream pool: the 32 streams of the
+    batch would otherwise shift the stream-to-queue mapping of the process).
+    At most C4_MAX_WARMUP + C4_MAX_STEPS iterations per design and phase, so
+    the measurement fits the bench whatever --steps is; the time limit scales
+    with them.  The child prints a Python traceback of every thread if it
+    crashes (faulthandler); on a time-out its output so far is kept."""
     import subprocess
-    cmd = [sys.executable, "-u", str(Path(__file__).resolve()), "--c4-only", "--precision", precision,
-           "--steps", str(steps), "--warmup", str(warmup)]
+    steps, warmup = min(steps, C4_MAX_STEPS), min(warmup, C4_MAX_WARMUP)
+    timeout_s = 180 + 40 * (steps + warmup)
+    cmd = [sys.executable, "-u", "-X", "faulthandler", str(Path(__file__).resolve()), "--c4-only",
+           "--precision", precision, "--steps", str(steps), "--warmup", str(warmup)]
     try:
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, cwd=str(ROOT))
-    except subprocess.TimeoutExpired:
-        return {"error": f"child timed out after {timeout_s}s"}
+    except subprocess.TimeoutExpired as e:
+        err = e.stderr.decode(errors="replace") if isinstance(e.stderr, bytes) else (e.stderr or "")
+        sys.stderr.write(err[-4000:])
+        return {"error": f"child timed out after {timeout_s}s", "stderr_tail": err[-1500:]}
     sys.stderr.write(r.stderr[-4000:])
     sys.stderr.flush()
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
     if r.returncode != 0 or not lines:
-        return {"error": f"child exited with status {r.returncode}", "stderr_tail": r.stderr[-600:]}
+        return {"error": f"child exited with status {r.returncode}", "stderr_tail": r.stderr[-1500:]}
     out = json.loads(lines[-1])
     out["process"] = "child"
     return out

@@ -19,7 +19,8 @@ from .fem import FEM3D, Material
 from .topopt import SIMPOptimizer
 from .workloads import E_PA, NU, RHO_MAT, Workload, load_cases, seeded_density
 
-__all__ = ["simp_for_workload", "make_designs", "step_designs", "run_designs", "histories"]
+__all__ = ["simp_for_workload", "make_designs", "step_designs", "run_designs", "close_designs",
+           "histories"]
 
 
 def simp_for_workload(wl: Workload, precision: str = "mixed", min_freq_Hz: float = 1e4,
@@ -52,6 +53,11 @@ class Design:
             for _ in range(n):
                 self.opt.step()
             self.stream.synchronize()
+
+    def close(self):
+        """Join the design's modal worker, then destroy its contexts."""
+        self.stream.synchronize()
+        self.opt.close(fem=True)
 
 
 def _map(fn, items, concurrent, max_workers):
@@ -92,16 +98,25 @@ def step_designs(designs, steps: int, concurrent: bool = True, max_workers: int 
     return designs
 
 
+def close_designs(designs):
+    """Release every design's worker thread and device contexts, one design
+    after the other, from the calling thread."""
+    for d in designs:
+        d.close()
+
+
 def run_designs(workloads, steps: int, precision: str = "mixed", concurrent: bool = True,
                 max_workers: int = 16):
     """Set up and run `steps` SIMP iterations of every workload; returns the
-    optimizers in input order."""
+    designs in input order (their optimisers are ``d.opt``; release them with
+    ``close_designs``)."""
     ds = make_designs(workloads, precision, concurrent, max_workers)
     step_designs(ds, steps, concurrent, max_workers)
-    return [d.opt for d in ds]
+    return ds
 
 
-def histories(opts):
+def histories(designs):
     """Compliance and first-frequency histories, (n_designs, steps) arrays."""
+    opts = [getattr(d, "opt", d) for d in designs]
     return (np.array([o.history["compliance"] for o in opts]),
             np.array([o.history["omega_1_Hz"] for o in opts]))
