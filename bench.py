@@ -18,10 +18,14 @@ Prints one JSON line.  Extra objects:
   solvers       the static (PCG) and modal (LOBPCG) solves alone on the final
                 design, cold-started: wall time, iterations, PCG iterations/s
   plain_fp64    one SIMP iteration of the same loop on the plain FP64 path
-                (the reference's Jacobi preconditioner in PCG and LOBPCG)
-  secondary     the other workloads given with --also (default c2, c3 and c4,
-                the 16 batched designs run concurrently and one by one, in a
-                child process)
+                (the reference's Jacobi preconditioner in PCG and LOBPCG),
+                resumed from the multigrid path's loop state at the start of
+                the timed window, beside the multigrid path's own time for
+                that iteration
+  secondary     the other workloads given with --also (default c2, c3, c5 --
+                12 modes ranked by z effective mass, at most 5 timed steps --
+                and c4, the 16 batched designs run concurrently and one by
+                one, at most 5 timed steps, in a child process)
 
 Other modes: --trajectory N (per-iteration times of both paths over N SIMP
 iterations), --kernels-only (kernel timings for counter runs), --c4-only.
@@ -62,15 +66,21 @@ def make_optimizer(wl_name, precision, modal_maxiter=2000):
                         r_min_mm=wl.r_min_vox * geom.dx * 1e3, max_iter=10 ** 6,
                         min_freq_Hz=1e4, tip_mass_kg=0.0, n_modes_track=wl.n_modes,
                         F_list=list(F), solve_rtol=wl.pcg_rtol, modal_tol=wl.modal_tol,
-                        modal_maxiter=modal_maxiter, solve_maxiter=40000)
+                        modal_maxiter=modal_maxiter, solve_maxiter=40000,
+                        effective_mass_direction=wl.eff_dir)
     opt.start(rho_init=x0)
     return wl, geom, fem, opt, F, x0
 
 
-def time_steps(opt, warmup, steps):
+def time_steps(opt, warmup, steps, snapshot=None):
+    """Time `steps` SIMP iterations after `warmup` untimed ones; with a
+    `snapshot` dict, the loop state at the start of the timed window is
+    stored in it (host copies, for the plain-path comparison)."""
     import torch
     for _ in range(warmup):
         opt.step()
+    if snapshot is not None:
+        snapshot.update(opt.state())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -246,12 +256,12 @@ def cpu_baseline(wl, geom, pcg_iters, lob_iters, basis, slab_nz=6, budget_s=25.0
             "seconds_per_iter": round(per_iter, 2)}
 
 
-def run_workload(name, precision, steps, warmup):
+def run_workload(name, precision, steps, warmup, snapshot=None):
     import numpy as np
     wl, geom, fem, opt, F, x0 = make_optimizer(name, precision)
     print(f"[bench] {wl.name}: {geom.nx}x{geom.ny}x{geom.nz}, {geom.n_dofs} DOF, {F.shape[0]} LC, "
           f"{wl.n_modes} modes ({precision}); warmup {warmup}", file=sys.stderr, flush=True)
-    dt = time_steps(opt, warmup, steps)
+    dt = time_steps(opt, warmup, steps, snapshot)
     h = opt.history
     pcg_it = float(np.mean([np.mean(v) for v in h["pcg_iters"][-steps:]]))
     lob_it = float(np.mean(h["lobpcg_iters"][-steps:]))
@@ -330,6 +340,8 @@ def _release(*objs):
 
 
 C4_MAX_STEPS, C4_MAX_WARMUP = 5, 1
+# secondary workloads measured on fewer iterations than the headline one (steps, warmup)
+SECONDARY_CAPS = {"c5": (5, 2)}
 
 
 def bench_c4(precision, steps, warmup):
@@ -394,12 +406,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c5t")
-    ap.add_argument("--also", default="c2,c3,c4",
+    ap.add_argument("--also", default="c2,c3,c5,c4",
                     help="comma-separated secondary workloads ('' for none; c4 = the 16 batched designs)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--plain-maxiter", type=int, default=2000,
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-kernels", action="store_true",
+                    help="skip the per-kernel HIP-event timings and the cold-start solver timings "
+                         "(for profiles of the timed window alone)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--trajectory", type=int, default=0, metavar="N",
                     help="instead of the bench line: N SIMP iterations of the workload on the "
@@ -441,12 +456,22 @@ def main():
                           "kernels": roofs}), flush=True)
         return
 
+    snap = {}
     wl, geom, fem, opt, dt, pcg_it, lob_it = run_workload(args.workload, args.precision, args.steps,
-                                                          args.warmup)
+                                                          args.warmup, snap)
     value = args.steps / dt
-    solvers = solver_timings(opt)
+    # the multigrid path's own time for the first timed iteration (the state
+    # the plain path is timed from below)
+    first_timed = {"ms": round(opt.history["time_s"][args.warmup] * 1e3, 2),
+                   "pcg_iters": opt.history["pcg_iters"][args.warmup],
+                   "lobpcg_iters": opt.history["lobpcg_iters"][args.warmup]}
+    solvers = None if args.no_kernels else solver_timings(opt)
     print(f"[bench] solves alone: {json.dumps(solvers)}", file=sys.stderr, flush=True)
-    roofs = kernel_rooflines(fem, geom, opt.rho_phys)
+    h = opt.history
+    for i in range(args.warmup, len(h["time_s"])):
+        print(f"[bench] iteration {i + 1}: {h['time_s'][i] * 1e3:.1f} ms, pcg {h['pcg_iters'][i]}, "
+              f"lobpcg {h['lobpcg_iters'][i]}", file=sys.stderr, flush=True)
+    roofs = {} if args.no_kernels else kernel_rooflines(fem, geom, opt.rho_phys)
     # Each workload's contexts are released before the next one starts: every
     # stream alive in the process shares the few hardware queues HIP maps
     # streams onto, and a context kept alive for nothing can make the next
@@ -459,11 +484,18 @@ def main():
     if run_c4:
         also.remove("c4")
     for name in also:
-        swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, args.steps, args.warmup)
-        secondary[name] = {"value": round(args.steps / sdt, 4), "unit": "SIMP iters/s",
-                           "ms_per_step": round(sdt / args.steps * 1e3, 2), "n_dofs": sgeom.n_dofs,
-                           "load_cases": len(swl.loads), "modes": swl.n_modes,
+        cap_s, cap_w = SECONDARY_CAPS.get(name, (args.steps, args.warmup))
+        ss, sw = min(args.steps, cap_s), min(args.warmup, cap_w)
+        swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, ss, sw)
+        secondary[name] = {"value": round(ss / sdt, 4), "unit": "SIMP iters/s",
+                           "ms_per_step": round(sdt / ss * 1e3, 2), "steps": ss, "warmup": sw,
+                           "n_dofs": sgeom.n_dofs, "load_cases": len(swl.loads), "modes": swl.n_modes,
                            "mean_pcg_iters": round(spcg, 1), "mean_lobpcg_iters": round(slob, 1)}
+        if swl.eff_dir is not None:
+            h = sopt.history
+            secondary[name]["effective_mass_direction"] = "xyz"[swl.eff_dir]
+            secondary[name]["last_eff_mass"] = [round(v, 6) for v in h["eff_mass"][-1]]
+            secondary[name]["last_mode_rank"] = h["mode_rank"][-1]
         _release(sopt)
         del sopt
     # The plain path runs after c2 and c3: every optimiser takes a fresh pooled
@@ -472,14 +504,25 @@ def main():
     # the static stream's queue (measured: c3 12.5 instead of 17.4 it/s).
     plain = None
     if not args.no_plain and args.precision != "fp64":
+        # one iteration of the plain path from the multigrid path's state at
+        # the start of the timed window (same design, same warm starts)
         _, _, _, popt, _, _ = make_optimizer(args.workload, "fp64", modal_maxiter=args.plain_maxiter)
+        popt.load_state(snap, overlap=True)
         pdt = time_steps(popt, 0, 1)
         ph = popt.history
+        lob = ph["lobpcg_iters"][-1]
         plain = {"value": round(1 / pdt, 4), "unit": "SIMP iters/s", "ms_per_step": round(pdt * 1e3, 2),
                  "steps": 1, "path": "FP64 Jacobi-PCG + Jacobi-LOBPCG (reference preconditioner)",
-                 "pcg_iters": ph["pcg_iters"][-1], "lobpcg_iters": ph["lobpcg_iters"][-1],
-                 "lobpcg_maxiter": args.plain_maxiter, "speedup_of_value": round(value * pdt, 2)}
-        print(f"[bench] plain FP64 path: {pdt:.3f}s per SIMP iteration", file=sys.stderr, flush=True)
+                 "state": f"the multigrid path's loop state after {args.warmup} iterations "
+                          "(iteration {} of the timed window's design)".format(args.warmup + 1),
+                 "pcg_iters": ph["pcg_iters"][-1], "lobpcg_iters": lob,
+                 "lobpcg_maxiter": args.plain_maxiter,
+                 "lobpcg_converged": bool(lob < args.plain_maxiter),
+                 "multigrid_same_iteration": first_timed,
+                 "speedup_same_iteration": round(pdt * 1e3 / first_timed["ms"], 2),
+                 "speedup_of_value": round(value * pdt, 2)}
+        print(f"[bench] plain FP64 path: {pdt:.3f}s for iteration {args.warmup + 1} "
+              f"(multigrid path {first_timed['ms'] / 1e3:.3f}s)", file=sys.stderr, flush=True)
         _release(popt)
         del popt
     if run_c4:
@@ -499,7 +542,7 @@ def main():
                    "n_dofs": geom.n_dofs, "pcg_rtol": wl.pcg_rtol, "lobpcg_tol": wl.modal_tol,
                    "mean_pcg_iters_per_lc": round(pcg_it, 1), "mean_lobpcg_iters": round(lob_it, 1),
                    "precision": args.precision},
-        "roofline": roofs[DOMINANT],
+        "roofline": roofs.get(DOMINANT),
         "kernels": {k: v for k, v in roofs.items() if k != DOMINANT},
         "solvers": solvers,
         "plain_fp64": plain,

@@ -130,7 +130,7 @@ int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, 
  * [lam_max/alpha, lam_max], coarsest-level size (DOFs) for the dense inverse,
  * cycle index (1 V-cycle, 2 W-cycle) applied on levels with at least
  * w_min_dofs DOFs (smaller levels take one coarse correction).  Defaults:
- * nu 2, alpha 20, coarse_dofs 700, cycle 2, w_min_dofs 5000. */
+ * nu 2, alpha 20, coarse_dofs 2000 (at most 4096), cycle 2, w_min_dofs 5000. */
 int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, int cycle,
                         int64_t w_min_dofs);
 /* Centre-point element stresses, Voigt order (xx, yy, zz, xy, yz, xz), linear
@@ -245,6 +245,17 @@ typedef struct vox8_tail_args {
   int k, have_prev;
 } vox8_tail_args;
 int vox8_simp_tail(vox8_ctx* ctx, const vox8_tail_args* args, double* h_state);
+
+/* Modal effective mass of k modes (columns of d_X, leading dimension ld,
+ * zero at fixed DOFs) in translation direction 0 (x), 1 (y) or 2 (z), with
+ * the mass matrix of the current density and tip lump: r = unit translation
+ * on the free DOFs,
+ *   h_frac[j] = (X_j^T M r)^2 / (X_j^T M X_j) / (r^T M r),
+ *   h_L[j] = X_j^T M r, h_gm[j] = X_j^T M X_j, *h_total = r^T M r
+ * (any output may be NULL), FP64 reductions on the device.  Replaces
+ * effective_mass_voxel, sims/modal_participation.py:37-57. */
+int vox8_effective_mass(vox8_ctx* ctx, int k, const double* d_X, int64_t ld, int direction,
+                        double* h_frac, double* h_L, double* h_gm, double* h_total);
 
 /* MFMA trial (no reference counterpart; a measurement, not used by the
  * solvers): d_Y = K d_X in FP32 for nrhs column-major vectors, the 24x24

@@ -11,7 +11,7 @@ each, run concurrently (one context, stream and thread per design) against
 the same designs run one after another: the histories must agree bitwise-close
 (deterministic kernels, independent state).  Each design's iteration-1
 compliance and first frequency must match a plain FP64 solve of that design
-alone (Jacobi PCG to rtol 1e-10, Jacobi LOBPCG to tol 1e-8, cold start) within
+alone (Jacobi PCG to rtol 1e-10, Jacobi LOBPCG to tol 1e-6, cold start) within
 1e-6 (sims/run_vf_gpu.py:133-141 runs each V_f as its own job).
 """
 import numpy as np
@@ -86,7 +86,7 @@ def test_c4_batched_designs_match_sequential_and_plain(gpu):
         U, info = fem.solve_block(rho, F, rtol=1e-10, maxiter=200000)
         assert info["info"] == [0]
         c = float(F[0] @ U[0].cpu().numpy())
-        lam, _, li = fem.modal_block(rho, wl.n_modes, 0.0, [], tol=1e-8, maxiter=40000)
+        lam, _, li = fem.modal_block(rho, wl.n_modes, 0.0, [], tol=1e-6, maxiter=40000)
         assert li["info"] == 0, (d, li)
         f1 = float(np.sqrt(np.min(lam)) / (2 * np.pi))
         fem.close()

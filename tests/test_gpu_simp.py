@@ -113,3 +113,30 @@ def test_auto_guard_columns_keep_the_history(gpu):
     a, b = runs[0]["history"], runs["auto"]["history"]
     for k in HIST:
         assert rel(np.asarray(b[k], dtype=float), np.asarray(a[k], dtype=float)) < 1e-6, k
+
+
+def test_checkpoint_resume_continues_the_run(gpu, tmp_path):
+    """save() after three iterations, load() into a fresh optimiser, two more
+    iterations: the same history as five uninterrupted iterations (the warm
+    starts, tracked mode and mu travel with the state)."""
+    from vox8.fem import FEM3D, Material
+    from vox8.topopt import SIMPOptimizer
+    geom, loads, opts, mat = tipbox_problem()
+    kw = dict(solve_rtol=TIPBOX_SOLVE_RTOL, modal_tol=TIPBOX_MODAL_TOL,
+              modal_maxiter=TIPBOX_MODAL_MAXITER, concurrent_solves=False, **opts)
+    a = SIMPOptimizer(geom, FEM3D(geom, Material(**mat), precision="mixed"), loads, **kw).start()
+    for _ in range(5):
+        a.step()
+    b = SIMPOptimizer(geom, FEM3D(geom, Material(**mat), precision="mixed"), loads, **kw).start()
+    for _ in range(3):
+        b.step()
+    path = tmp_path / "state.npz"
+    b.save(path)
+    c = SIMPOptimizer(geom, FEM3D(geom, Material(**mat), precision="mixed"), loads, **kw).start()
+    c.load(path)
+    assert c.it == 3
+    for _ in range(2):
+        c.step()
+    for k in ("compliance", "change", "omega_1_Hz", "mu_freq"):
+        assert rel(np.asarray(c.history[k]), np.asarray(a.history[k][3:])) < 1e-6, k
+    assert rel(c.x.cpu().numpy(), a.x.cpu().numpy()) < 1e-6

@@ -76,6 +76,9 @@ struct vox8_ctx {
   int lobpcg_guard = -1;            // guard columns of vox8_lobpcg (-1: environment or 0)
   double *d_kphi = nullptr, *d_mphi = nullptr, *d_ce = nullptr, *d_sums = nullptr,
          *d_wts = nullptr;
+  // effective mass: M [X, r] and r (em_cols columns of n_dofs)
+  double* d_em = nullptr;
+  int em_cols = 0;
 };
 
 namespace {
@@ -290,7 +293,8 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   VOX8_CHECK((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (int64_t)1 << 31, "grid too large");
   vox8_ctx* c = new vox8_ctx();
   c->g = make_grid(nx, ny, nz);
-  if (const char* e = getenv("VOX8_MG_WMIN")) c->mg.w_min_dofs = atoll(e);   // tuning override
+  if (const char* e = getenv("VOX8_MG_WMIN")) c->mg.w_min_dofs = atoll(e);   // tuning overrides
+  if (const char* e = getenv("VOX8_MG_COARSE")) c->mg.coarse_dofs = std::max(24, std::min(4096, atoi(e)));
   c->st = (hipStream_t)hip_stream;
   int s = 0;
   auto fail = [&](int code) { vox8_destroy(c); *out = nullptr; return code; };
@@ -422,9 +426,9 @@ int vox8_mg_info(vox8_ctx* c, int* h_nlevels, int* h_dims, double* h_lam_max, in
 int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs, int cycle,
                         int64_t w_min_dofs) {
   VOX8_CHECK(c != nullptr, "null context");
-  VOX8_CHECK(nu >= 1 && nu <= 16 && alpha > 1.0 && coarse_dofs >= 24 && coarse_dofs <= 1000 &&
+  VOX8_CHECK(nu >= 1 && nu <= 16 && alpha > 1.0 && coarse_dofs >= 24 && coarse_dofs <= 4096 &&
                  cycle >= 1 && cycle <= 2 && w_min_dofs >= 0,
-             "mg options out of range (nu 1..16, alpha > 1, coarse_dofs 24..1000, cycle 1 or 2, "
+             "mg options out of range (nu 1..16, alpha > 1, coarse_dofs 24..4096, cycle 1 or 2, "
              "w_min_dofs >= 0)");
   c->mg.nu = nu;
   c->mg.gamma = cycle;
@@ -831,6 +835,40 @@ int vox8_simp_tail(vox8_ctx* c, const vox8_tail_args* a, double* h_state) {
   VOX8_TRY(launch_stats(ne, a->d_design, a->d_xnew, nullptr, c->red_partials, c->red_counters,
                         state + 15, c->st));
   return sync_read(c, state, h_state, TAIL_STATE);
+  API_END
+}
+
+int vox8_effective_mass(vox8_ctx* c, int k, const double* d_X, int64_t ld, int direction,
+                        double* h_frac, double* h_L, double* h_gm, double* h_total) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_X && k >= 1 && k <= 16, "effective mass: 1..16 modes");
+  VOX8_CHECK(ld >= c->g.n_dofs, "ld < n_dofs");
+  VOX8_CHECK(direction >= 0 && direction <= 2, "direction must be 0 (x), 1 (y) or 2 (z)");
+  const int64_t nd = c->g.n_dofs;
+  if (c->em_cols < k + 2) {
+    dfree(c, c->d_em);
+    c->d_em = nullptr;
+    c->em_cols = 0;
+    VOX8_TRY(dalloc(c, &c->d_em, (size_t)(k + 2) * nd));
+    c->em_cols = k + 2;
+  }
+  double* MX = c->d_em;
+  double* r = c->d_em + (int64_t)(k + 1) * nd;
+  VOX8_TRY(launch_unit_translation(c->g.n_nodes, direction, c->d_fixed, r, c->st));
+  VOX8_TRY(apply_op<double>(c, OP_M, d_X, ld, nullptr, MX, nd, k, DotFuse()));
+  VOX8_TRY(apply_op<double>(c, OP_M, r, nd, nullptr, MX + (int64_t)k * nd, nd, 1, DotFuse()));
+  VOX8_TRY(launch_effective_mass(c->g.n_nodes, direction, k, d_X, ld, MX, c->red_partials,
+                                 4096 * 40, c->red_out, c->st));
+  double h[64];
+  VOX8_TRY(sync_read(c, c->red_out, h, 3 * k + 1));
+  for (int j = 0; j < k; ++j) {
+    if (h_frac) h_frac[j] = h[j];
+    if (h_L) h_L[j] = h[k + j];
+    if (h_gm) h_gm[j] = h[2 * k + j];
+  }
+  if (h_total) *h_total = h[3 * k];
+  return OK;
   API_END
 }
 

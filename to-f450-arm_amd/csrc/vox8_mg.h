@@ -52,11 +52,12 @@ struct MultiGrid {
   int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
   int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when
                                // gamma = 2: their kernels are launch-latency bound
-  int coarse_dofs = 700;       // stop coarsening at this many DOFs (dense inverse)
+  int coarse_dofs = 2000;      // stop coarsening at this many DOFs (dense inverse)
   double alpha = 20.0;         // smoothing interval [lam_max/alpha, lam_max]
   // coarsest level: dense inverse when small
   int n_coarse = 0;
-  double* A = nullptr;         // n_coarse^2 FP64 (assembled, then inverted in place)
+  double* A = nullptr;         // n_coarse^2 FP64 (assembled, then inverted)
+  double* A2 = nullptr;        // second n_coarse^2 buffer of the Gauss-Jordan ping-pong
   float* Ainv = nullptr;       // FP32 copy used in the V-cycle
   double* red = nullptr;       // reduction scratch
   unsigned int* counter = nullptr;

@@ -190,45 +190,36 @@ __global__ void k_mg_coarse_assemble(Grid g, const double* __restrict__ f, const
   }
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one pivot
-// per launch, every entry of the n x n matrix updated in parallel:
-//   p = 1/a_kk; a_kj *= p (j != k); a_ij -= a_ik a_kj (i, j != k);
+// Gauss-Jordan inverse of an SPD matrix (no pivoting), one pivot per
+// launch, every entry of the n x n matrix updated in parallel from the
+// previous iterate into the other buffer of a pair (no read/write hazard):
+//   p = 1/a_kk; a_kj *= p (j != k); a_ij -= a_ik a_kj p (i, j != k);
 //   a_ik = -a_ik p (i != k); a_kk = p.
-// The pivot row is read in its pre-step state from `rowk`, the pivot column
-// from `colk` (both copied by k_gj_stage), so the update has no read/write hazard.
 // The pivot of step k is the Schur-complement diagonal, positive for an SPD
 // matrix; a pivot below 1e-10 of the original diagonal marks the coarse matrix
 // singular (flag = 1; step 0 also clears the flag).
-__global__ void k_gj_stage(int n, int k, const double* __restrict__ A, double* __restrict__ rowk,
-                           double* __restrict__ colk, const double* __restrict__ diag0,
-                           double* __restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  rowk[i] = A[(int64_t)k * n + i];
-  colk[i] = A[(int64_t)i * n + k];
-  if (i == k) {
-    const bool bad = !(A[(int64_t)k * n + k] > 1e-10 * diag0[k]);
-    if (k == 0) *flag = bad ? 1.0 : 0.0;
-    else if (bad) *flag = 1.0;
-  }
-}
-
 __global__ void k_gj_diag(int n, const double* __restrict__ A, double* __restrict__ diag0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) diag0[i] = fabs(A[(int64_t)i * n + i]);
 }
 
-__global__ void k_gj_step(int n, int k, double* __restrict__ A, const double* __restrict__ rowk,
-                          const double* __restrict__ colk) {
+__global__ void k_gj_pivot(int n, int k, const double* __restrict__ A, double* __restrict__ B,
+                           const double* __restrict__ diag0, double* __restrict__ flag) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)n * n) return;
   const int i = (int)(e / n), j = (int)(e % n);
-  const double p = 1.0 / rowk[k];
+  const double akk = A[(int64_t)k * n + k];
+  const double p = 1.0 / akk;
   double v;
-  if (i == k) v = (j == k) ? p : rowk[j] * p;
-  else if (j == k) v = -colk[i] * p;
-  else v = A[e] - colk[i] * rowk[j] * p;
-  A[e] = v;
+  if (i == k) v = (j == k) ? p : A[(int64_t)k * n + j] * p;
+  else if (j == k) v = -A[(int64_t)i * n + k] * p;
+  else v = A[e] - A[(int64_t)i * n + k] * A[(int64_t)k * n + j] * p;
+  B[e] = v;
+  if (e == 0) {
+    const bool bad = !(akk > 1e-10 * diag0[k]);
+    if (k == 0) *flag = bad ? 1.0 : 0.0;
+    else if (bad) *flag = 1.0;
+  }
 }
 
 __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -497,13 +488,16 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
     scale *= 2.0;
   }
   const MGLevel& C = mg.lv.back();
-  mg.n_coarse = C.g.n_dofs <= 700 ? (int)C.g.n_dofs : 0;
+  mg.n_coarse = C.g.n_dofs <= mg.coarse_dofs ? (int)C.g.n_dofs : 0;
   void* p = nullptr;
   if (mg.n_coarse) {
-    VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * mg.n_coarse * mg.n_coarse)); mg.A = (double*)p;
-    VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * mg.n_coarse * mg.n_coarse)); mg.Ainv = (float*)p;
+    const size_t nn = (size_t)mg.n_coarse * mg.n_coarse;
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A = (double*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A2 = (double*)p;
+    VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * nn)); mg.Ainv = (float*)p;
   }
-  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 6144)); mg.red = (double*)p;
+  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * std::max<size_t>(6144, (size_t)mg.n_coarse + 8)));
+  mg.red = (double*)p;
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(unsigned int) * 4)); mg.counter = (unsigned int*)p;
   VOX8_HIP(hipMemsetAsync(mg.counter, 0, sizeof(unsigned int) * 4, st));
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * MG_SCAL_SIZE)); mg.scal = (double*)p;
@@ -610,19 +604,18 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
     hipLaunchKernelGGL(k_mg_coarse_assemble, dim3(nb((int64_t)n * n, 4096)), dim3(256), 0, st, C.g,
                        (const double*)(mg.lv.size() == 1 ? fK0 : C.fK64), (const uint8_t*)C.fixed, dKe,
                        C.scale, mg.A);
-    double* rowk = mg.red;
-    double* colk = mg.red + 2048;
-    double* diag0 = mg.red + 4096;
+    double* diag0 = mg.red;
     const unsigned gs = (unsigned)((n + 255) / 256), ge = (unsigned)(((int64_t)n * n + 255) / 256);
     hipLaunchKernelGGL(k_gj_diag, dim3(gs), dim3(256), 0, st, n, (const double*)mg.A, diag0);
+    double* a = mg.A;
+    double* b = mg.A2;
     for (int k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_gj_stage, dim3(gs), dim3(256), 0, st, n, k, (const double*)mg.A, rowk, colk,
+      hipLaunchKernelGGL(k_gj_pivot, dim3(ge), dim3(256), 0, st, n, k, (const double*)a, b,
                          (const double*)diag0, mg.scal + 7);
-      hipLaunchKernelGGL(k_gj_step, dim3(ge), dim3(256), 0, st, n, k, mg.A, (const double*)rowk,
-                         (const double*)colk);
+      std::swap(a, b);
     }
     hipLaunchKernelGGL(k_to_f32_n, dim3(nb((int64_t)n * n)), dim3(256), 0, st, (int64_t)n * n,
-                       (const double*)mg.A, mg.Ainv);
+                       (const double*)a, mg.Ainv);
     VOX8_HIP(hipGetLastError());
     VOX8_HIP(hipMemcpyAsync(mg.host + 7, mg.scal + 7, sizeof(double), hipMemcpyDeviceToHost, st));
     VOX8_HIP(hipStreamSynchronize(st));

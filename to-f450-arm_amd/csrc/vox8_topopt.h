@@ -43,4 +43,13 @@ int launch_tail_penalty(int64_t n, double mu, double* state, const double* x, do
                         hipStream_t st);
 int launch_compose(int64_t n, const uint8_t* design, const uint8_t* in_arm, const double* filt,
                    double* out, hipStream_t st);
+// Modal effective mass: r = free-DOF unit translation; reductions of
+// MX = M [X_0..X_{k-1}, r] (k+1 columns, ld 3 n_nodes) into out[3k+1]
+// (fractions, L_j = X_j^T M r, g_j = X_j^T M X_j, r^T M r).  `partials`
+// holds max_doubles doubles.
+int launch_unit_translation(int64_t n_nodes, int dir, const uint8_t* fixed, double* r,
+                            hipStream_t st);
+int launch_effective_mass(int64_t n_nodes, int dir, int k, const double* X, int64_t ldx,
+                          const double* MX, double* partials, int max_doubles, double* out,
+                          hipStream_t st);
 }  // namespace vox8

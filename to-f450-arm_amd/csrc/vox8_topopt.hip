@@ -502,3 +502,85 @@ int launch_compose(int64_t n, const uint8_t* design, const uint8_t* in_arm, cons
 }
 
 }  // namespace vox8
+
+// ---- modal effective mass (sims/modal_participation.py:37-57) ------------------
+namespace vox8 {
+
+// r = unit rigid translation in direction dir on the free DOFs (zero at fixed DOFs)
+__global__ void k_unit_translation(int64_t n_nodes, int dir, const uint8_t* __restrict__ fixed,
+                                   double* __restrict__ r) {
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t f = fixed[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[3 * n + c] = (c == dir && !((f >> c) & 1)) ? 1.0 : 0.0;
+  }
+}
+
+// Block partials, column j = blockIdx.y of MX (ld = 3 n_nodes, zero at fixed
+// DOFs): L_j = sum of its direction-dir entries (= X_j^T M r, M symmetric,
+// r the free-DOF translation) and, for j < k, g_j = X_j . (M X_j).  Column k
+// holds M r, whose L is r^T M r.
+__global__ void __launch_bounds__(256)
+k_em_partials(int64_t n_nodes, int dir, int k, const double* __restrict__ X, int64_t ldx,
+              const double* __restrict__ MX, double* __restrict__ partials) {
+  __shared__ double sh[8];
+  const int col = blockIdx.y;
+  const double* y = MX + (int64_t)col * 3 * n_nodes;
+  const double* x = col < k ? X + (int64_t)col * ldx : nullptr;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < n_nodes;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const double y0 = y[3 * n], y1 = y[3 * n + 1], y2 = y[3 * n + 2];
+    s1 += dir == 0 ? y0 : (dir == 1 ? y1 : y2);
+    if (x) s2 += x[3 * n] * y0 + x[3 * n + 1] * y1 + x[3 * n + 2] * y2;
+  }
+  double t[2];
+  t[0] = block_sum(s1, sh);
+  t[1] = block_sum(s2, sh);
+  write_partials<2>(t, partials + (int64_t)col * gridDim.x * 2, blockIdx.x);
+}
+
+// out[j] = L_j^2 / max(g_j, 1e-30) / (r^T M r), out[k + j] = L_j,
+// out[2k + j] = g_j, out[3k] = r^T M r; fixed-order sums (one block).
+__global__ void __launch_bounds__(256)
+k_em_finish(int nb, int k, const double* __restrict__ partials, double* __restrict__ out) {
+  __shared__ double sh[8];
+  double tot[2];
+  sum_partials<2>(partials + (int64_t)k * nb * 2, nb, tot, sh);
+  const double total = tot[0];
+  for (int j = 0; j < k; ++j) {
+    double t[2];
+    sum_partials<2>(partials + (int64_t)j * nb * 2, nb, t, sh);
+    if (threadIdx.x == 0) {
+      out[j] = total > 0.0 ? t[0] * t[0] / fmax(t[1], 1e-30) / total : 0.0;
+      out[k + j] = t[0];
+      out[2 * k + j] = t[1];
+    }
+  }
+  if (threadIdx.x == 0) out[3 * k] = total;
+}
+
+int launch_unit_translation(int64_t n_nodes, int dir, const uint8_t* fixed, double* r,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_unit_translation, dim3(nblk(n_nodes, 4096)), dim3(256), 0, st, n_nodes, dir,
+                     fixed, r);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+int launch_effective_mass(int64_t n_nodes, int dir, int k, const double* X, int64_t ldx,
+                          const double* MX, double* partials, int max_doubles, double* out,
+                          hipStream_t st) {
+  VOX8_CHECK(k >= 1 && k <= 16 && dir >= 0 && dir <= 2, "effective mass: 1..16 modes, direction 0..2");
+  int nb = (int)nblk(n_nodes, 512);
+  if ((int64_t)nb * 2 * (k + 1) > max_doubles) nb = max_doubles / (2 * (k + 1));
+  VOX8_CHECK(nb >= 1, "effective mass: reduction workspace too small");
+  hipLaunchKernelGGL(k_em_partials, dim3(nb, k + 1), dim3(256), 0, st, n_nodes, dir, k, X, ldx, MX,
+                     partials);
+  hipLaunchKernelGGL(k_em_finish, dim3(1), dim3(256), 0, st, nb, k, (const double*)partials, out);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace vox8

@@ -25,6 +25,7 @@ import torch
 from . import _lib
 from ._lib import call, ptr
 from .fem import FEM3D, _tip_nodes_of
+from .modal_participation import effective_mass_device, rank_by_effective_mass
 
 import ctypes as C
 
@@ -109,7 +110,11 @@ class SIMPOptimizer:
     modal_guard (guard columns of the modal solve: 0..2, or "auto": none until
     an iteration's LOBPCG needs more than GUARD_TRIGGER iterations, then two;
     late in a run the last tracked mode's neighbour comes close and the last
-    pair then converges slowly without them).
+    pair then converges slowly without them), effective_mass_direction (None,
+    or 0/1/2: after each modal solve the tracked modes' effective-mass
+    fractions in that translation direction, sims/modal_participation.py:37-57,
+    are computed on the device and the history gains "eff_mass" (per mode, in
+    frequency order) and "mode_rank" (mode indices by decreasing fraction)).
     """
 
     GUARD_TRIGGER = 50
@@ -119,7 +124,7 @@ class SIMPOptimizer:
                  n_modes_track=4, mu_freq_init=0.10, mu_freq_max=50.0, mu_freq_ramp_every=5,
                  mu_freq_ramp_factor=1.5, F_list=None, solve_rtol=None, modal_tol=1e-4,
                  modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True, device_tail=True,
-                 modal_guard="auto"):
+                 modal_guard="auto", effective_mass_direction=None):
         self.geom, self.fem, self.load_cases = geom, fem, load_cases
         self.vol_frac = vol_frac
         self.r_min_vox = max(1.0, r_min_mm / (geom.dx * 1e3))
@@ -137,6 +142,9 @@ class SIMPOptimizer:
         if modal_guard != "auto" and int(modal_guard) not in (0, 1, 2):
             raise ValueError("modal_guard must be 'auto' or 0..2")
         self.modal_guard = modal_guard
+        if effective_mass_direction not in (None, 0, 1, 2):
+            raise ValueError("effective_mass_direction must be None, 0, 1 or 2")
+        self.eff_dir = effective_mass_direction
         self._guard = 0 if modal_guard == "auto" else int(modal_guard)
         self._guard_set = None
         self._fem_modal = None
@@ -220,7 +228,7 @@ class SIMPOptimizer:
         self.w_target = (2.0 * np.pi * self.min_freq_Hz) ** 2 if self.modal else None
         self.history = {k: [] for k in ("compliance", "mass", "change", "vol_frac", "omega_1_Hz",
                                         "mu_freq", "freq_violation", "pcg_iters", "lobpcg_iters",
-                                        "time_s")}
+                                        "time_s", "eff_mass", "mode_rank")}
         self.it = 0
         return self
 
@@ -253,6 +261,8 @@ class SIMPOptimizer:
                     res = self._fem_modal.modal_block(rho_phys, self.n_modes_track, self.tip_mass_kg,
                                                       self.tip, X0=X0, tol=self.modal_tol,
                                                       maxiter=self.modal_maxiter)
+                    if self.eff_dir is not None:
+                        res = res + (effective_mass_device(self._fem_modal, res[1], self.eff_dir)[0],)
                     done = torch.cuda.Event()
                     done.record()
                 return res, done
@@ -274,8 +284,11 @@ class SIMPOptimizer:
         lob_iters = 0
         lam = X = None
         if self.modal:
+            em = None
             if modal_job is not None:
-                (lam, X, linfo), done = modal_job.result()
+                res, done = modal_job.result()
+                lam, X, linfo = res[:3]
+                em = res[3] if len(res) > 3 else None
                 cur = torch.cuda.current_stream()
                 cur.wait_event(done)
                 X.record_stream(cur)
@@ -283,6 +296,8 @@ class SIMPOptimizer:
                 lam, X, linfo = fem.modal_block(rho_phys, self.n_modes_track, self.tip_mass_kg,
                                                 self.tip, X0=self.X, tol=self.modal_tol,
                                                 maxiter=self.modal_maxiter)
+                if self.eff_dir is not None:
+                    em = effective_mass_device(fem, X, self.eff_dir)[0]
             lob_iters = linfo["iters"]
             if self.modal_guard == "auto" and lob_iters > self.GUARD_TRIGGER and self._guard != 2:
                 self._guard = 2
@@ -294,6 +309,10 @@ class SIMPOptimizer:
             if not np.array_equal(order, np.arange(len(order))):
                 X = X[torch.as_tensor(order, device=fem.device)].contiguous()
             self.X = X
+            if em is not None:
+                em = em[order]
+                self.history["eff_mass"].append([float(v) for v in em])
+                self.history["mode_rank"].append([int(v) for v in rank_by_effective_mass(em)])
         if self.device_tail:
             change, vol = self._device_tail(lam, X, it)
         else:
@@ -411,6 +430,58 @@ class SIMPOptimizer:
         _, _, change, _ = self._stats(self._check, self.phys_prev)
         vol, _, _, _ = self._stats(self._xnew, mask=self._design)
         return change, vol
+
+    # ------------------------------------------------------------------ checkpoint
+    def state(self):
+        """The loop state after the last step, as host arrays: design
+        variables and the previous filtered design, the warm starts of both
+        solves (U, X), the tracked mode, mu, the iteration count and the
+        guard setting.  The reference keeps the same warm starts in memory
+        (src/topopt.py:237, 246, 288-290, 323); ``load_state`` resumes from it,
+        in this optimiser or another one on the same problem (e.g. the plain
+        FP64 path from the multigrid path's state)."""
+        st = {"x": self.x.cpu().numpy(), "phys_prev": self.phys_prev.cpu().numpy(),
+              "it": np.int64(self.it), "mu_freq": np.float64(self.mu_freq),
+              "guard": np.int64(self._guard)}
+        if self.U is not None:
+            st["U"] = self.U.cpu().numpy()
+        if self.X is not None:
+            st["X"] = self.X.cpu().numpy()
+        if self.v_prev is not None:
+            st["v_prev"] = self.v_prev.cpu().numpy()
+        return st
+
+    def load_state(self, st, overlap=False):
+        """Resume from ``state()`` (or ``np.load`` of ``save``); call after
+        start().  overlap=True lets the next step overlap the two solves at
+        once (otherwise, as after start(), the first step runs them one after
+        the other so the multigrid launch shapes are tuned on an idle GPU)."""
+        dev = self.fem.device
+        if overlap and getattr(self, "_fem_modal", None) is not None:
+            self._overlap = True
+        self.x = torch.as_tensor(np.asarray(st["x"]), device=dev).clone()
+        self.phys_prev = torch.as_tensor(np.asarray(st["phys_prev"]), device=dev).clone()
+        self.it = int(st["it"])
+        self.mu_freq = float(st["mu_freq"])
+        if self.modal_guard == "auto":
+            self._guard = int(st["guard"])
+        self.U = torch.as_tensor(np.asarray(st["U"]), device=dev).clone() if "U" in st else None
+        self.X = torch.as_tensor(np.asarray(st["X"]), device=dev).clone() if "X" in st else None
+        if "v_prev" in st:
+            self._vprev.copy_(torch.as_tensor(np.asarray(st["v_prev"]), device=dev))
+            self.v_prev = self._vprev
+        else:
+            self.v_prev = None
+        return self
+
+    def save(self, path):
+        """Checkpoint the loop state to an .npz file (no pickled objects)."""
+        np.savez(path, **self.state())
+
+    def load(self, path):
+        """Resume from a checkpoint written by ``save``."""
+        with np.load(path, allow_pickle=False) as f:
+            return self.load_state({k: f[k] for k in f.files})
 
     def result(self):
         fem, g = self.fem, self.geom
