@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 DOMINANT = "cheb_fp32"
 
 
-def make_optimizer(wl_name, precision, modal_maxiter=2000):
+def make_optimizer(wl_name, precision, modal_maxiter=2000, galerkin=True, modal_guard=0):
     from vox8.fem import FEM3D, Material
     from vox8.topopt import SIMPOptimizer
     from vox8.workloads import E_PA, NU, RHO_MAT, WORKLOADS, load_cases, seeded_density
@@ -59,6 +59,8 @@ def make_optimizer(wl_name, precision, modal_maxiter=2000):
     wl = WORKLOADS[wl_name]
     geom = wl.geometry()
     fem = FEM3D(geom, Material(E=E_PA, nu=NU, rho=RHO_MAT), precision=precision)
+    if not galerkin:
+        fem.set_mg_galerkin(False)
     F = load_cases(geom, wl.loads)
     x0 = seeded_density(geom, wl.base, wl.amp, wl.seed)
     # frequency target above the initial f1 keeps the eigen-adjoint active
@@ -67,7 +69,7 @@ def make_optimizer(wl_name, precision, modal_maxiter=2000):
                         min_freq_Hz=1e4, tip_mass_kg=0.0, n_modes_track=wl.n_modes,
                         F_list=list(F), solve_rtol=wl.pcg_rtol, modal_tol=wl.modal_tol,
                         modal_maxiter=modal_maxiter, solve_maxiter=40000,
-                        effective_mass_direction=wl.eff_dir)
+                        effective_mass_direction=wl.eff_dir, modal_guard=modal_guard)
     opt.start(rho_init=x0)
     return wl, geom, fem, opt, F, x0
 
@@ -79,12 +81,14 @@ def time_steps(opt, warmup, steps, snapshot=None):
     import torch
     for _ in range(warmup):
         opt.step()
+        print(f"[bench] warmup step {opt.it} done", file=sys.stderr, flush=True)
     if snapshot is not None:
         snapshot.update(opt.state())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         opt.step()
+        print(f"[bench] step {opt.it} done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     return time.perf_counter() - t0
 
@@ -256,9 +260,10 @@ def cpu_baseline(wl, geom, pcg_iters, lob_iters, basis, slab_nz=6, budget_s=25.0
             "seconds_per_iter": round(per_iter, 2)}
 
 
-def run_workload(name, precision, steps, warmup, snapshot=None):
+def run_workload(name, precision, steps, warmup, snapshot=None, galerkin=True, modal_guard=0):
     import numpy as np
-    wl, geom, fem, opt, F, x0 = make_optimizer(name, precision)
+    wl, geom, fem, opt, F, x0 = make_optimizer(name, precision, galerkin=galerkin,
+                                               modal_guard=modal_guard)
     print(f"[bench] {wl.name}: {geom.nx}x{geom.ny}x{geom.nz}, {geom.n_dofs} DOF, {F.shape[0]} LC, "
           f"{wl.n_modes} modes ({precision}); warmup {warmup}", file=sys.stderr, flush=True)
     dt = time_steps(opt, warmup, steps, snapshot)
@@ -412,6 +417,10 @@ def main():
     ap.add_argument("--plain-maxiter", type=int, default=2000,
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--modal-guard", default="0", choices=["auto", "0", "1", "2"],
+                    help="LOBPCG guard columns of the SIMP loop's modal solve")
+    ap.add_argument("--coarse", default="galerkin", choices=["galerkin", "rediscretised"],
+                    help="coarse levels of the multigrid preconditioner")
     ap.add_argument("--no-kernels", action="store_true",
                     help="skip the per-kernel HIP-event timings and the cold-start solver timings "
                          "(for profiles of the timed window alone)")
@@ -458,7 +467,8 @@ def main():
 
     snap = {}
     wl, geom, fem, opt, dt, pcg_it, lob_it = run_workload(args.workload, args.precision, args.steps,
-                                                          args.warmup, snap)
+                                                          args.warmup, snap, args.coarse == "galerkin",
+                                                          args.modal_guard)
     value = args.steps / dt
     # the multigrid path's own time for the first timed iteration (the state
     # the plain path is timed from below)
@@ -486,7 +496,8 @@ def main():
     for name in also:
         cap_s, cap_w = SECONDARY_CAPS.get(name, (args.steps, args.warmup))
         ss, sw = min(args.steps, cap_s), min(args.warmup, cap_w)
-        swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, ss, sw)
+        swl, sgeom, _, sopt, sdt, spcg, slob = run_workload(name, args.precision, ss, sw,
+                                                            galerkin=args.coarse == "galerkin")
         secondary[name] = {"value": round(ss / sdt, 4), "unit": "SIMP iters/s",
                            "ms_per_step": round(sdt / ss * 1e3, 2), "steps": ss, "warmup": sw,
                            "n_dofs": sgeom.n_dofs, "load_cases": len(swl.loads), "modes": swl.n_modes,
@@ -541,7 +552,8 @@ def main():
                                f"{geom.dx * 1e3:g} mm, {len(wl.loads)} load cases, {wl.n_modes} modes",
                    "n_dofs": geom.n_dofs, "pcg_rtol": wl.pcg_rtol, "lobpcg_tol": wl.modal_tol,
                    "mean_pcg_iters_per_lc": round(pcg_it, 1), "mean_lobpcg_iters": round(lob_it, 1),
-                   "precision": args.precision},
+                   "precision": args.precision,
+                   "multigrid_coarse_levels": args.coarse},
         "roofline": roofs.get(DOMINANT),
         "kernels": {k: v for k, v in roofs.items() if k != DOMINANT},
         "solvers": solvers,

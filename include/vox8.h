@@ -133,6 +133,15 @@ int vox8_mg_info(vox8_ctx* ctx, int* h_nlevels, int* h_dims, double* h_lam_max, 
  * nu 2, alpha 20, coarse_dofs 2000 (at most 4096), cycle 2, w_min_dofs 5000. */
 int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, int cycle,
                         int64_t w_min_dofs);
+/* Coarse levels of the multigrid preconditioner: 1 = Galerkin
+ * A_{l+1} = P^T A_l P (per-element 24x24 matrices, applied in node-block
+ * form; the default), 0 = rediscretised (the box element of each level with
+ * the mean of its children's factors, applied by the matrix-free structured
+ * stencil: cheaper passes, but at high density contrast the cycle can lose
+ * its effect and PCG stalls).  No reference counterpart (the reference
+ * preconditions with the Jacobi diagonal). */
+int vox8_set_mg_galerkin(vox8_ctx* ctx, int enable);
+
 /* Centre-point element stresses, Voigt order (xx, yy, zz, xy, yz, xz), linear
  * in the density: d_stress[6e + i] = rho[e] (D B u_e)_i for every element of
  * the grid (0 on inactive elements).  h_B is the 6x24 centre

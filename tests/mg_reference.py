@@ -8,6 +8,8 @@ levels coarsened 2:1 (ceil), factor = mean of 8 children, element matrix
 is fixed when any fine node of its 3x3x3 interpolation stencil has it fixed), R = P^T, Chebyshev
 smoothing of D^-1 K on [lam_max/alpha, lam_max], dense coarsest solve; cycle
 index gamma (1 = V, 2 = W with a fresh residual after each coarse correction).
+galerkin=True replaces the rediscretised coarse operators by P^T K P (sparse
+matrix products), the device's Galerkin option (vox8_set_mg_galerkin).
 """
 from __future__ import annotations
 

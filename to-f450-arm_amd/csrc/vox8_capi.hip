@@ -295,6 +295,10 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   c->g = make_grid(nx, ny, nz);
   if (const char* e = getenv("VOX8_MG_WMIN")) c->mg.w_min_dofs = atoll(e);   // tuning overrides
   if (const char* e = getenv("VOX8_MG_COARSE")) c->mg.coarse_dofs = std::max(24, std::min(4096, atoi(e)));
+  if (const char* e = getenv("VOX8_MG_GALERKIN")) c->mg.galerkin = e[0] != '0';
+  if (const char* e = getenv("VOX8_MG_NU")) c->mg.nu = std::max(1, std::min(16, atoi(e)));
+  if (const char* e = getenv("VOX8_MG_ALPHA")) c->mg.alpha = std::max(1.5, atof(e));
+  if (const char* e = getenv("VOX8_MG_MARGIN")) c->mg.lam_margin = std::max(1.0, atof(e));
   c->st = (hipStream_t)hip_stream;
   int s = 0;
   auto fail = [&](int code) { vox8_destroy(c); *out = nullptr; return code; };
@@ -438,6 +442,16 @@ int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs, int 
   c->mg.coarse_dofs = coarse_dofs;
   c->mg_version = ~0ull;
   ++c->mg.epoch;
+  return OK;
+}
+
+int vox8_set_mg_galerkin(vox8_ctx* c, int enable) {
+  VOX8_CHECK(c != nullptr, "null context");
+  if (c->mg.galerkin != (enable != 0)) {
+    c->mg.galerkin = enable != 0;
+    c->mg.built = false;           // the level operators change form
+    c->mg_version = ~0ull;
+  }
   return OK;
 }
 

@@ -28,6 +28,8 @@ struct MGLevel {
   float* dinv = nullptr;       // Jacobi inverse diagonal
   double lam_max = 2.0;        // estimate of lambda_max(D^-1 K)
   float* pv = nullptr;         // Lanczos vector (the other one is d)
+  double* E64 = nullptr;       // Galerkin levels: per-element 24x24 matrices (FP64)
+  float* S32 = nullptr;        //   and the FP32 node blocks the smoother applies
   // work vectors, ncols column-major blocks of n_dofs
   float *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d2 = nullptr;
 };
@@ -54,6 +56,8 @@ struct MultiGrid {
                                // gamma = 2: their kernels are launch-latency bound
   int coarse_dofs = 2000;      // stop coarsening at this many DOFs (dense inverse)
   double alpha = 20.0;         // smoothing interval [lam_max/alpha, lam_max]
+  bool galerkin = true;        // coarse levels P^T A P (element matrices); false: rediscretised
+  double lam_margin = 1.1;     // lambda_max = margin x the largest Lanczos Ritz value
   // coarsest level: dense inverse when small
   int n_coarse = 0;
   double* A = nullptr;         // n_coarse^2 FP64 (assembled, then inverted)
@@ -80,6 +84,18 @@ struct MultiGrid {
   MultiGrid& operator=(const MultiGrid&) = delete;
   ~MultiGrid();
 };
+
+// Galerkin coarse levels (vox8_mg_galerkin.hip).  gal_build_level: element
+// matrices of coarse grid gc from fine grid gf (fK0/dKe for the finest
+// level, else the fine level's element matrices Ef64).
+int gal_build_level(const Grid& gf, const Grid& gc, const double* fK0, const double* dKe,
+                    const double* Ef64, const uint8_t* fixf, const uint8_t* fixc, double* Ec64,
+                    float* S32, hipStream_t st);
+int gal_jacobi(const Grid& g, const double* E64, const uint8_t* fixed, float* dinv, hipStream_t st);
+int gal_coarse_assemble(const Grid& g, const double* E64, const uint8_t* fixed, double* A,
+                        hipStream_t st);
+int gal_apply(const Grid& g, const float* S32, const uint8_t* fixed, const float* x, float* y,
+              int ncols, const Epilogue<float>& ep, hipStream_t st);
 
 // Destroy every captured cycle (device state is unchanged).
 void mg_drop_graphs(MultiGrid& mg);

@@ -190,36 +190,89 @@ __global__ void k_mg_coarse_assemble(Grid g, const double* __restrict__ f, const
   }
 }
 
-// Gauss-Jordan inverse of an SPD matrix (no pivoting), one pivot per
-// launch, every entry of the n x n matrix updated in parallel from the
-// previous iterate into the other buffer of a pair (no read/write hazard):
-//   p = 1/a_kk; a_kj *= p (j != k); a_ij -= a_ik a_kj p (i, j != k);
-//   a_ik = -a_ik p (i != k); a_kk = p.
-// The pivot of step k is the Schur-complement diagonal, positive for an SPD
-// matrix; a pivot below 1e-10 of the original diagonal marks the coarse matrix
-// singular (flag = 1; step 0 also clears the flag).
+// Blocked Gauss-Jordan inverse of an SPD matrix (no pivoting).  For each
+// block K of GJ_B consecutive pivots, with R the other indices and
+// D = A_KK^-1:
+//   A_KK <- D,  A_KR <- D A_KR,  A_RK <- -A_RK D,  A_RR <- A_RR - A_RK D A_KR,
+// the scalar Gauss-Jordan step applied to a block; after the last block A
+// holds its inverse.  Three launches per block: D by an in-LDS scalar
+// Gauss-Jordan in one workgroup, W = A_:K D, then every entry from the
+// previous iterate into the other buffer of a pair (no read/write hazard).
+// The scalar pivots are the Schur-complement diagonals, positive for an SPD
+// matrix; one below 1e-10 of the original diagonal marks the coarse matrix
+// singular (flag = 1; the first block clears the flag).
+constexpr int GJ_B = 32;
+
 __global__ void k_gj_diag(int n, const double* __restrict__ A, double* __restrict__ diag0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) diag0[i] = fabs(A[(int64_t)i * n + i]);
 }
 
-__global__ void k_gj_pivot(int n, int k, const double* __restrict__ A, double* __restrict__ B,
-                           const double* __restrict__ diag0, double* __restrict__ flag) {
+__global__ void __launch_bounds__(256)
+k_gjb_panel(int n, int k0, int b, const double* __restrict__ A, double* __restrict__ D,
+            const double* __restrict__ diag0, double* __restrict__ flag) {
+  __shared__ double m[2][GJ_B * GJ_B];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  for (int e = threadIdx.x; e < b * b; e += blockDim.x)
+    m[0][e] = A[(int64_t)(k0 + e / b) * n + k0 + e % b];
+  __syncthreads();
+  int cur = 0;
+  for (int p = 0; p < b; ++p) {
+    const double* a = m[cur];
+    double* o = m[cur ^ 1];
+    const double app = a[p * b + p];
+    const double q = 1.0 / app;
+    for (int e = threadIdx.x; e < b * b; e += blockDim.x) {
+      const int i = e / b, j = e % b;
+      double v;
+      if (i == p) v = (j == p) ? q : a[p * b + j] * q;
+      else if (j == p) v = -a[i * b + p] * q;
+      else v = a[e] - a[i * b + p] * a[p * b + j] * q;
+      o[e] = v;
+    }
+    if (threadIdx.x == 0 && !(app > 1e-10 * diag0[k0 + p])) bad = 1;
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int e = threadIdx.x; e < b * b; e += blockDim.x) D[e] = m[cur][e];
+  if (threadIdx.x == 0) {
+    if (k0 == 0) *flag = bad ? 1.0 : 0.0;
+    else if (bad) *flag = 1.0;
+  }
+}
+
+// W[i][l] = sum_m A[i][k0 + m] D[m][l]
+__global__ void k_gjb_w(int n, int k0, int b, const double* __restrict__ A, const double* __restrict__ D,
+                        double* __restrict__ W) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * b) return;
+  const int i = (int)(e / b), l = (int)(e % b);
+  const double* ar = A + (int64_t)i * n + k0;
+  double s = 0.0;
+  for (int m = 0; m < b; ++m) s += ar[m] * D[m * b + l];
+  W[e] = s;
+}
+
+__global__ void k_gjb_update(int n, int k0, int b, const double* __restrict__ A, double* __restrict__ B,
+                             const double* __restrict__ D, const double* __restrict__ W) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)n * n) return;
   const int i = (int)(e / n), j = (int)(e % n);
-  const double akk = A[(int64_t)k * n + k];
-  const double p = 1.0 / akk;
+  const bool iK = i >= k0 && i < k0 + b, jK = j >= k0 && j < k0 + b;
   double v;
-  if (i == k) v = (j == k) ? p : A[(int64_t)k * n + j] * p;
-  else if (j == k) v = -A[(int64_t)i * n + k] * p;
-  else v = A[e] - A[(int64_t)i * n + k] * A[(int64_t)k * n + j] * p;
-  B[e] = v;
-  if (e == 0) {
-    const bool bad = !(akk > 1e-10 * diag0[k]);
-    if (k == 0) *flag = bad ? 1.0 : 0.0;
-    else if (bad) *flag = 1.0;
+  if (iK && jK) {
+    v = D[(i - k0) * b + (j - k0)];
+  } else if (iK) {
+    v = 0.0;
+    for (int m = 0; m < b; ++m) v += D[(i - k0) * b + m] * A[(int64_t)(k0 + m) * n + j];
+  } else if (jK) {
+    v = -W[(int64_t)i * b + (j - k0)];
+  } else {
+    v = A[e];
+    for (int m = 0; m < b; ++m) v -= W[(int64_t)i * b + m] * A[(int64_t)(k0 + m) * n + j];
   }
+  B[e] = v;
 }
 
 __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -228,30 +281,35 @@ __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __res
     b[i] = (float)a[i];
 }
 
-// x = Ainv b per column (Ainv symmetric).  Each wavefront owns 4 rows; its
-// lanes stride over the n columns (coalesced row reads), then reduce across
-// the wavefront.  16 rows per block, fixed summation order.
+// x = Ainv b for up to CN columns at once (blockIdx.y selects the column
+// group), so the inverse is read once per group.  One row per wavefront (its
+// lanes stride over the row: coalesced reads; the columns of b come from
+// cache), reduced across the wavefront; 4 rows per block, fixed summation
+// order.
+template <int CN>
 __global__ void __launch_bounds__(256)
-k_mg_coarse_apply(int n, const float* __restrict__ Ainv, const float* __restrict__ b,
+k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float* __restrict__ b,
                   float* __restrict__ x, const int* __restrict__ skip) {
   const int lane = threadIdx.x & 63;
-  const int row0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
-  const int col = blockIdx.y;
-  if (skip && skip[col]) return;
-  const float* bc = b + (int64_t)col * n;
-  float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k = lane; k < n; k += 64) {
-    const float bk = bc[k];
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int c0 = blockIdx.y * CN;
+  float s[CN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (row0 + i < n) s[i] += Ainv[(int64_t)(row0 + i) * n + k] * bk;
+  for (int c = 0; c < CN; ++c) s[c] = 0.0f;
+  const float* ar = Ainv + (int64_t)row * n;
+  for (int k = lane; k < n; k += 64) {
+    const float a = ar[k];
+#pragma unroll
+    for (int c = 0; c < CN; ++c) s[c] += a * (c0 + c < ncols ? b[(int64_t)(c0 + c) * n + k] : 0.0f);
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float v = s[i];
+  for (int c = 0; c < CN; ++c) {
+    float v = s[c];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0 && row0 + i < n) x[(int64_t)col * n + row0 + i] = v;
+    const int col = c0 + c;
+    if (lane == 0 && col < ncols && !(skip && skip[col])) x[(int64_t)col * n + row] = v;
   }
 }
 
@@ -404,6 +462,7 @@ static double tridiag_max_eig(const double* a, const double* b, int m) {
 // ---------------------------------------------------------------------------
 static int level_apply(const MGLevel& L, const float* x, float* y, int ncols, hipStream_t st,
                        Epilogue<float> ep) {
+  if (L.S32) return gal_apply(L.g, L.S32, L.fixed, x, y, ncols, ep, st);
   return launch_apply_fast<float>(L.g, L.pattern, OP_K, L.c32, L.fK, nullptr, L.fixed, x,
                                   L.g.n_dofs, y, nullptr, L.g.n_dofs, ncols, st, DotFuse(), ep);
 }
@@ -440,6 +499,10 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
       VOX8_HIP(hipMemcpyAsync(L.active, act.data(), g.n_elem, hipMemcpyHostToDevice, st));
       VOX8_HIP(hipMemcpyAsync(L.fixed, fix.data(), g.n_nodes, hipMemcpyHostToDevice, st));
       VOX8_HIP(hipStreamSynchronize(st));   // act / fix change on the next level
+      if (mg.galerkin) {
+        VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 576 * g.n_elem)); L.E64 = (double*)p;
+        VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * 128 * g.n_nodes)); L.S32 = (float*)p;
+      }
     }
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.dinv = (float*)p;
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.pv = (float*)p;
@@ -496,7 +559,8 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A2 = (double*)p;
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * nn)); mg.Ainv = (float*)p;
   }
-  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * std::max<size_t>(6144, (size_t)mg.n_coarse + 8)));
+  VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * std::max<size_t>(6144, (size_t)mg.n_coarse * (GJ_B + 1)
+                                                                       + GJ_B * GJ_B + 8)));
   mg.red = (double*)p;
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(unsigned int) * 4)); mg.counter = (unsigned int*)p;
   VOX8_HIP(hipMemsetAsync(mg.counter, 0, sizeof(unsigned int) * 4, st));
@@ -530,6 +594,13 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
   for (size_t l = 0; l < mg.lv.size(); ++l) {
     MGLevel& L = mg.lv[l];
     const double* f64 = l == 0 ? fK0 : L.fK64;
+    if (l > 0 && L.E64) {
+      const MGLevel& F = mg.lv[l - 1];
+      VOX8_TRY(gal_build_level(F.g, L.g, l == 1 ? fK0 : nullptr, dKe, l == 1 ? nullptr : F.E64,
+                               F.fixed, L.fixed, L.E64, L.S32, st));
+      VOX8_TRY(gal_jacobi(L.g, L.E64, L.fixed, L.dinv, st));
+      continue;
+    }
     if (l > 0) {
       const MGLevel& F = mg.lv[l - 1];
       hipLaunchKernelGGL(k_mg_factors, dim3(nb(L.g.n_elem, 1 << 20)), dim3(256), 0, st, F.g, L.g,
@@ -594,24 +665,34 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
       // the Krylov space is exhausted (levels with fewer free DOFs than steps)
       if (!(b[j] > 1e-7 * amax)) break;
     }
-    L.lam_max = 1.1 * tridiag_max_eig(a, b, m);
+    L.lam_max = mg.lam_margin * tridiag_max_eig(a, b, m);
     if (!(L.lam_max > 0.0) || !std::isfinite(L.lam_max)) L.lam_max = 2.5;
   }
   // coarsest level: assemble and invert
   if (mg.n_coarse) {
     const MGLevel& C = mg.lv.back();
     const int n = mg.n_coarse;
-    hipLaunchKernelGGL(k_mg_coarse_assemble, dim3(nb((int64_t)n * n, 4096)), dim3(256), 0, st, C.g,
-                       (const double*)(mg.lv.size() == 1 ? fK0 : C.fK64), (const uint8_t*)C.fixed, dKe,
-                       C.scale, mg.A);
+    if (C.E64)
+      VOX8_TRY(gal_coarse_assemble(C.g, C.E64, C.fixed, mg.A, st));
+    else
+      hipLaunchKernelGGL(k_mg_coarse_assemble, dim3(nb((int64_t)n * n, 4096)), dim3(256), 0, st, C.g,
+                         (const double*)(mg.lv.size() == 1 ? fK0 : C.fK64), (const uint8_t*)C.fixed, dKe,
+                         C.scale, mg.A);
     double* diag0 = mg.red;
     const unsigned gs = (unsigned)((n + 255) / 256), ge = (unsigned)(((int64_t)n * n + 255) / 256);
     hipLaunchKernelGGL(k_gj_diag, dim3(gs), dim3(256), 0, st, n, (const double*)mg.A, diag0);
     double* a = mg.A;
     double* b = mg.A2;
-    for (int k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_gj_pivot, dim3(ge), dim3(256), 0, st, n, k, (const double*)a, b,
+    double* D = mg.red + n;                       // GJ_B^2 doubles
+    double* W = mg.red + n + GJ_B * GJ_B;         // n * GJ_B doubles
+    for (int k0 = 0; k0 < n; k0 += GJ_B) {
+      const int bs = std::min(GJ_B, n - k0);
+      hipLaunchKernelGGL(k_gjb_panel, dim3(1), dim3(256), 0, st, n, k0, bs, (const double*)a, D,
                          (const double*)diag0, mg.scal + 7);
+      hipLaunchKernelGGL(k_gjb_w, dim3((unsigned)(((int64_t)n * bs + 255) / 256)), dim3(256), 0, st, n,
+                         k0, bs, (const double*)a, (const double*)D, W);
+      hipLaunchKernelGGL(k_gjb_update, dim3(ge), dim3(256), 0, st, n, k0, bs, (const double*)a, b,
+                         (const double*)D, (const double*)W);
       std::swap(a, b);
     }
     hipLaunchKernelGGL(k_to_f32_n, dim3(nb((int64_t)n * n)), dim3(256), 0, st, (int64_t)n * n,
@@ -675,8 +756,13 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
-    hipLaunchKernelGGL(k_mg_coarse_apply, dim3((mg.n_coarse + 15) / 16, ncols), dim3(256), 0, st,
-                       mg.n_coarse, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
+    const unsigned rb = (unsigned)((mg.n_coarse + 3) / 4);
+    if (ncols <= 4)
+      hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, 1), dim3(256), 0, st, mg.n_coarse, ncols,
+                         (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
+    else
+      hipLaunchKernelGGL(k_mg_coarse_apply<8>, dim3(rb, (ncols + 7) / 8), dim3(256), 0, st,
+                         mg.n_coarse, ncols, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
     VOX8_HIP(hipGetLastError());
     return OK;
   }
@@ -810,6 +896,7 @@ int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t s
       const size_t nsm = mg.n_coarse ? mg.lv.size() - 1 : mg.lv.size();
       for (size_t l = 0; l < nsm; ++l) {
         MGLevel& L = mg.lv[l];
+        if (L.S32) continue;                  // node-block levels have no shape to tune
         VOX8_TRY(stencil_tune_fp32(L.g, L.pattern, L.c32, L.fK, L.fixed, L.dinv, L.d, L.r, L.x,
                                    L.d2, L.g.n_dofs, ncols, st));
       }

@@ -1,0 +1,351 @@
+// vox8 — Galerkin coarse operators for the multigrid preconditioner.
+//
+// A_{l+1} = P^T A_l P with the masked trilinear transfer of vox8_mg.hip.  On
+// a 2:1-coarsened box grid every fine element lies inside one coarse element
+// and its nodal values interpolate from that element's 8 corners only, so
+// the triple product is element-local: the coarse element matrix is
+//   E_c = sum over the (up to 8) children f of  P_f^T E_f P_f,
+// with P_f the 24x24 interpolation of child f's DOFs from the parent's
+// corners (weight per axis 1, 1/2 or 0), E_f = f_e Ke on the fine level
+// (rows/columns of fixed DOFs zeroed) or the child's Galerkin matrix, and the
+// rows/columns of the coarse fixed DOFs zeroed.  The element matrices (FP64)
+// build the next level and the coarsest dense matrix; the smoother applies
+// each coarse level in node-block form (the 27-point 3x3-block stencil,
+// FP32, symmetric half stored) with the Chebyshev epilogues of the
+// structured stencil.
+#include "vox8_mg.h"
+
+namespace vox8 {
+
+// weight of parent corner A for the local node a of child c (per axis: the
+// node sits at half-position p = child bit + node offset in {0, 1, 2}; the
+// corner at 2 q, q in {0, 1})
+__device__ __forceinline__ double gal_w(int c, int a, int A) {
+  double w = 1.0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int cb = ax == 0 ? (c & 1) : (ax == 1 ? ((c >> 1) & 1) : (c >> 2));
+    const int p = cb + local_off(a, ax);
+    const int q = local_off(A, ax);
+    w *= (p == 2 * q) ? 1.0 : (p == 1 ? 0.5 : 0.0);
+  }
+  return w;
+}
+
+__device__ __forceinline__ int64_t gal_node(const Grid& g, int ex, int ey, int ez, int a) {
+  return (ex + local_off(a, 0)) + (int64_t)g.NX * ((ey + local_off(a, 1)) + (int64_t)g.NY * (ez + local_off(a, 2)));
+}
+
+// one workgroup per coarse element; FROM_FACTORS: children are fine-level
+// elements f_e Ke (masked by the fine fixed DOFs), else stored child matrices
+template <bool FROM_FACTORS>
+__global__ void __launch_bounds__(256)
+k_gal_build(Grid gf, Grid gc, const double* __restrict__ fK, const double* __restrict__ Ke,
+            const double* __restrict__ Ef, const uint8_t* __restrict__ fixf,
+            const uint8_t* __restrict__ fixc, double* __restrict__ Ec64) {
+  __shared__ double acc[576], kc[576], tt[576];
+  __shared__ uint8_t fxl[8];
+  const int64_t E = blockIdx.x;
+  const int X = (int)(E % gc.nx), Y = (int)((E / gc.nx) % gc.ny), Z = (int)(E / ((int64_t)gc.nx * gc.ny));
+  for (int i = threadIdx.x; i < 576; i += blockDim.x) acc[i] = 0.0;
+  for (int c = 0; c < 8; ++c) {
+    const int fx = 2 * X + (c & 1), fy = 2 * Y + ((c >> 1) & 1), fz = 2 * Z + (c >> 2);
+    if (fx >= gf.nx || fy >= gf.ny || fz >= gf.nz) continue;
+    const int64_t f = fx + (int64_t)gf.nx * (fy + (int64_t)gf.ny * fz);
+    double fe = 1.0;
+    if (FROM_FACTORS) {
+      fe = fK[f];
+      if (fe == 0.0) continue;                 // inactive child: no contribution
+      if (threadIdx.x < 8) fxl[threadIdx.x] = fixf[gal_node(gf, fx, fy, fz, threadIdx.x)];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 576; i += blockDim.x) {
+      if (FROM_FACTORS) {
+        const int r = i / 24, q = i % 24;
+        const bool m = ((fxl[r / 3] >> (r % 3)) & 1) || ((fxl[q / 3] >> (q % 3)) & 1);
+        kc[i] = m ? 0.0 : fe * Ke[i];
+      } else {
+        kc[i] = Ef[f * 576 + i];
+      }
+    }
+    __syncthreads();
+    // T = E_f P_c: T[i][3A+cA] = sum_b E_f[i][3b+cA] w(c, b, A)
+    for (int i = threadIdx.x; i < 576; i += blockDim.x) {
+      const int r = i / 24, J = i % 24, A = J / 3, cA = J % 3;
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) s += kc[r * 24 + 3 * b + cA] * gal_w(c, b, A);
+      tt[i] = s;
+    }
+    __syncthreads();
+    // acc += P_c^T T: acc[3A+cI][J] += sum_a w(c, a, A) T[3a+cI][J]
+    for (int i = threadIdx.x; i < 576; i += blockDim.x) {
+      const int I = i / 24, J = i % 24, A = I / 3, cI = I % 3;
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 8; ++a) s += gal_w(c, a, A) * tt[(3 * a + cI) * 24 + J];
+      acc[i] += s;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) fxl[threadIdx.x] = fixc[gal_node(gc, X, Y, Z, threadIdx.x)];
+  __syncthreads();
+  for (int i = threadIdx.x; i < 576; i += blockDim.x) {
+    const int r = i / 24, q = i % 24;
+    const bool m = ((fxl[r / 3] >> (r % 3)) & 1) || ((fxl[q / 3] >> (q % 3)) & 1);
+    Ec64[E * 576 + i] = m ? 0.0 : acc[i];
+  }
+}
+
+// Jacobi inverse diagonal of an element-assembled level
+__global__ void k_gal_jacobi(Grid g, const double* __restrict__ E64, const uint8_t* __restrict__ fixed,
+                             float* __restrict__ dinv) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
+  double d[3] = {0, 0, 0};
+  for (int s = 0; s < 8; ++s) {
+    const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
+    const int ex = ix - 1 + sx, ey = iy - 1 + sy, ez = iz - 1 + sz;
+    if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+    const int64_t e = ex + (int64_t)g.nx * (ey + (int64_t)g.ny * ez);
+    const int a = local_idx(1 - sx, 1 - sy, 1 - sz);
+    for (int c = 0; c < 3; ++c) d[c] += E64[e * 576 + (3 * a + c) * 25];
+  }
+  const uint8_t fx = fixed[n];
+  for (int c = 0; c < 3; ++c)
+    dinv[3 * n + c] = ((fx >> c) & 1) ? 0.0f : (d[c] != 0.0 ? (float)(1.0 / d[c]) : 1.0f);
+}
+
+// dense matrix of an element-assembled level (identity rows/columns at fixed DOFs)
+__global__ void k_gal_coarse_assemble(Grid g, const double* __restrict__ E64,
+                                      const uint8_t* __restrict__ fixed, double* __restrict__ A) {
+  const int64_t n = g.n_dofs;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e % n;
+    const int64_t ni = i / 3, nj = j / 3;
+    const int ci = (int)(i % 3), cj = (int)(j % 3);
+    if (((fixed[ni] >> ci) & 1) || ((fixed[nj] >> cj) & 1)) { A[e] = (i == j) ? 1.0 : 0.0; continue; }
+    const int ix = (int)(ni % g.NX), iy = (int)((ni / g.NX) % g.NY), iz = (int)(ni / ((int64_t)g.NX * g.NY));
+    const int jx = (int)(nj % g.NX), jy = (int)((nj / g.NX) % g.NY), jz = (int)(nj / ((int64_t)g.NX * g.NY));
+    double s = 0.0;
+    for (int q = 0; q < 8; ++q) {
+      const int sx = q & 1, sy = (q >> 1) & 1, sz = q >> 2;
+      const int ex = ix - 1 + sx, ey = iy - 1 + sy, ez = iz - 1 + sz;
+      if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+      const int ox = jx - ex, oy = jy - ey, oz = jz - ez;
+      if (ox < 0 || ox > 1 || oy < 0 || oy > 1 || oz < 0 || oz > 1) continue;
+      const int a = local_idx(1 - sx, 1 - sy, 1 - sz), b = local_idx(ox, oy, oz);
+      s += E64[(ex + (int64_t)g.nx * (ey + (int64_t)g.ny * ez)) * 576 + (3 * a + ci) * 24 + 3 * b + cj];
+    }
+    A[e] = s;
+  }
+}
+
+// Node-block form of an element-assembled level: per node i the 3x3 blocks
+// A(i, i) and A(i, i + d) for the 13 "forward" offsets d (positive linear
+// offset); A(i, i - d) = A(i - d, i)^T is read from node i - d.  Stored FP32,
+// node-major S[i * 128 + blk * 9 + 3 r + c] (126 values padded to 128, half
+// the 27-point stencil's 243): the lanes working on one node read one
+// contiguous 512-byte record.
+__constant__ int GAL_D[14][3] = {{0, 0, 0},
+                                 {1, 0, 0},
+                                 {-1, 1, 0}, {0, 1, 0}, {1, 1, 0},
+                                 {-1, -1, 1}, {0, -1, 1}, {1, -1, 1},
+                                 {-1, 0, 1}, {0, 0, 1}, {1, 0, 1},
+                                 {-1, 1, 1}, {0, 1, 1}, {1, 1, 1}};
+
+__global__ void k_gal_nodeblocks(Grid g, const double* __restrict__ E64, float* __restrict__ S) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
+  for (int blk = 0; blk < 14; ++blk) {
+    const int dx = GAL_D[blk][0], dy = GAL_D[blk][1], dz = GAL_D[blk][2];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < 8; ++s) {
+      const int sx = s & 1, sy = (s >> 1) & 1, sz = s >> 2;
+      const int ex = ix - 1 + sx, ey = iy - 1 + sy, ez = iz - 1 + sz;
+      if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+      const int ox = 1 - sx + dx, oy = 1 - sy + dy, oz = 1 - sz + dz;   // neighbour in the element
+      if (ox < 0 || ox > 1 || oy < 0 || oy > 1 || oz < 0 || oz > 1) continue;
+      const int a = local_idx(1 - sx, 1 - sy, 1 - sz), b = local_idx(ox, oy, oz);
+      const double* Ee = E64 + (ex + (int64_t)g.nx * (ey + (int64_t)g.ny * ez)) * 576;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[3 * r + c] += Ee[(3 * a + r) * 24 + 3 * b + c];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) S[n * 128 + blk * 9 + q] = (float)acc[q];
+  }
+}
+
+// y = A x for a node-block level with the Chebyshev/residual epilogues of
+// the structured stencil (Epilogue in vox8_fast.h).  Eight lanes per node
+// share its 27 block products (the diagonal block, 13 forward blocks stored
+// at the node, 13 backward blocks A(n - d, n)^T stored at the neighbour):
+// lane l takes products l, l + 8, ... for all NC columns of the group
+// (blockIdx.y), so every block is loaded once, and the partial sums meet in
+// a butterfly over the 8 lanes; lane l then runs the epilogue of column l.
+// Workgroups are renumbered so that each XCD (they take workgroups
+// round-robin) works on a contiguous range of nodes, keeping the y and z
+// neighbours' blocks and vectors in its own L2.  The blocks are zero at
+// fixed DOFs, so x needs no masking and the product is zero there.
+template <int MODE, int NC>
+__global__ void __launch_bounds__(256)
+k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
+            const uint8_t* __restrict__ fixed, const float* __restrict__ x, float* __restrict__ y,
+            Epilogue<float> ep) {
+  constexpr int NPW = 256 / 8;                         // nodes per workgroup
+  const int wg = (blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
+  const int64_t n = (int64_t)wg * NPW + threadIdx.x / 8;
+  const int l8 = threadIdx.x & 7;
+  const int c0 = blockIdx.y * NC;
+  const bool live = n < g.n_nodes;                    // whole 8-lane groups share this
+  const int64_t ld = g.n_dofs, nn = g.n_nodes;
+  const int64_t nc = live ? n : 0;
+  const int ix = (int)(nc % g.NX), iy = (int)((nc / g.NX) % g.NY), iz = (int)(nc / ((int64_t)g.NX * g.NY));
+  const float c0f = ep.c0;
+  float acc[NC][3];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
+  if (live) {
+    for (int t = l8; t < 27; t += 8) {
+      const int blk = t < 14 ? t : t - 13;
+      const int sgn = t < 14 ? 1 : -1;
+      const int jx = ix + sgn * GAL_D[blk][0], jy = iy + sgn * GAL_D[blk][1], jz = iz + sgn * GAL_D[blk][2];
+      if (jx < 0 || jy < 0 || jz < 0 || jx >= g.NX || jy >= g.NY || jz >= g.NZ) continue;
+      const int64_t j = jx + (int64_t)g.NX * (jy + (int64_t)g.NY * jz);
+      const float* bp = S + (t < 14 ? n : j) * 128 + blk * 9;
+      float b[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) b[q] = bp[q];
+      if (t >= 14) {                                   // backward: the transpose
+        float u;
+        u = b[1]; b[1] = b[3]; b[3] = u;
+        u = b[2]; b[2] = b[6]; b[6] = u;
+        u = b[5]; b[5] = b[7]; b[7] = u;
+      }
+      float s0 = 1.0f, s1 = 1.0f, s2 = 1.0f;
+      if (MODE == EPI_CHEB0) {                         // x holds b; the input is c0 D^-1 b
+        s0 = c0f * ep.dinv[3 * j];
+        s1 = c0f * ep.dinv[3 * j + 1];
+        s2 = c0f * ep.dinv[3 * j + 2];
+      }
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const int col = c0 + k < ncols ? c0 + k : ncols - 1;
+        const float* xc = x + (int64_t)col * ld + 3 * j;
+        const float x0 = xc[0] * s0, x1 = xc[1] * s1, x2 = xc[2] * s2;
+        acc[k][0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
+        acc[k][1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
+        acc[k][2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
+      }
+    }
+  }
+  // sum over the node's 8 lanes (aligned groups of 8 within the wavefront)
+#pragma unroll
+  for (int k = 0; k < NC; ++k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float v = acc[k][c];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      acc[k][c] = v;
+    }
+  if (!live || l8 >= NC) return;
+  const int col = c0 + l8;
+  if (col >= ncols || (ep.skip && ep.skip[col])) return;
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+  for (int k = 0; k < NC; ++k)
+    if (k == l8) { a0 = acc[k][0]; a1 = acc[k][1]; a2 = acc[k][2]; }
+  const uint8_t fxn = fixed[n];
+  const int64_t o = (int64_t)col * ld + 3 * n;
+  const float av[3] = {a0, a1, a2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const bool fx = (fxn >> c) & 1;
+    const float val = fx ? 0.0f : av[c];
+    if (MODE == EPI_PLAIN) {
+      y[o + c] = val;
+    } else if (MODE == EPI_RESID) {
+      const float r = fx ? 0.0f : ep.b[o + c] - val;
+      ep.r[o + c] = r;
+      ep.dnew[o + c] = ep.c2 * ep.dinv[3 * n + c] * r;
+    } else if (MODE == EPI_CHEB) {
+      const float d = x[o + c];
+      ep.xs[o + c] += d;
+      const float r = ep.r[o + c] - val;
+      ep.r[o + c] = r;
+      ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+    } else {                                               // EPI_CHEB0 (x holds b)
+      const float bv = x[o + c];
+      const float d = c0f * ep.dinv[3 * n + c] * bv;
+      ep.xs[o + c] = d;
+      const float r = fx ? 0.0f : bv - val;
+      ep.r[o + c] = r;
+      ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+    }
+  }
+}
+
+int gal_build_level(const Grid& gf, const Grid& gc, const double* fK0, const double* dKe,
+                    const double* Ef64, const uint8_t* fixf, const uint8_t* fixc, double* Ec64,
+                    float* S32, hipStream_t st) {
+  if (Ef64)
+    hipLaunchKernelGGL(k_gal_build<false>, dim3((unsigned)gc.n_elem), dim3(256), 0, st, gf, gc,
+                       (const double*)nullptr, (const double*)nullptr, Ef64, fixf, fixc, Ec64);
+  else
+    hipLaunchKernelGGL(k_gal_build<true>, dim3((unsigned)gc.n_elem), dim3(256), 0, st, gf, gc, fK0,
+                       dKe, (const double*)nullptr, fixf, fixc, Ec64);
+  hipLaunchKernelGGL(k_gal_nodeblocks, dim3((unsigned)((gc.n_nodes + 255) / 256)), dim3(256), 0, st, gc,
+                     (const double*)Ec64, S32);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+int gal_jacobi(const Grid& g, const double* E64, const uint8_t* fixed, float* dinv, hipStream_t st) {
+  hipLaunchKernelGGL(k_gal_jacobi, dim3((unsigned)((g.n_nodes + 255) / 256)), dim3(256), 0, st, g, E64,
+                     fixed, dinv);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+int gal_coarse_assemble(const Grid& g, const double* E64, const uint8_t* fixed, double* A,
+                        hipStream_t st) {
+  const int64_t nn = g.n_dofs * g.n_dofs;
+  int64_t b = (nn + 255) / 256;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_gal_coarse_assemble, dim3((unsigned)b), dim3(256), 0, st, g, E64, fixed, A);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+template <int NC>
+static void gal_apply_nc(const Grid& g, const float* S32, const uint8_t* fixed, const float* x,
+                         float* y, int ncols, const Epilogue<float>& ep, hipStream_t st) {
+  const int nwg = (int)((g.n_nodes + 31) / 32);       // 32 nodes per workgroup
+  const int per_xcd = (nwg + 7) / 8;
+  const dim3 grid((unsigned)(8 * per_xcd), (unsigned)((ncols + NC - 1) / NC)), blk(256);
+  switch (ep.mode) {
+    case EPI_RESID: hipLaunchKernelGGL((k_gal_apply<EPI_RESID, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
+    case EPI_CHEB: hipLaunchKernelGGL((k_gal_apply<EPI_CHEB, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
+    case EPI_CHEB0: hipLaunchKernelGGL((k_gal_apply<EPI_CHEB0, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
+    default: hipLaunchKernelGGL((k_gal_apply<EPI_PLAIN, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
+  }
+}
+
+int gal_apply(const Grid& g, const float* S32, const uint8_t* fixed, const float* x, float* y,
+              int ncols, const Epilogue<float>& ep, hipStream_t st) {
+  if (ncols <= 1) gal_apply_nc<1>(g, S32, fixed, x, y, ncols, ep, st);
+  else if (ncols <= 2) gal_apply_nc<2>(g, S32, fixed, x, y, ncols, ep, st);
+  else if (ncols <= 4) gal_apply_nc<4>(g, S32, fixed, x, y, ncols, ep, st);
+  else gal_apply_nc<8>(g, S32, fixed, x, y, ncols, ep, st);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
+}  // namespace vox8

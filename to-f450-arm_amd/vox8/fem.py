@@ -181,6 +181,7 @@ class FEM3D:
         self.free_dofs = np.nonzero(free)[0]
         self.rho_active = np.ones(self.n_active)
         self._elem_node_ids = None
+        self.mg_galerkin = True
         # device context
         self.device = _device()
         self._stream = torch.cuda.current_stream(self.device).cuda_stream
@@ -412,12 +413,23 @@ class FEM3D:
         element matrices, SIMP constants) on the caller's current stream, e.g.
         to run the modal solve concurrently with the static one."""
         t = FEM3D(self.geom, self.mat, precision=self.precision)
+        if not self.mg_galerkin:
+            t.set_mg_galerkin(False)
         t.Ke_solid = np.array(self.Ke_solid, copy=True)
         t.Me_solid = np.array(self.Me_solid, copy=True)
         for k in ("SIMP_PENALTY", "E_MIN_FRAC", "MASS_THRESHOLD", "MASS_PENALTY_Q"):
             if k in self.__dict__:
                 setattr(t, k, getattr(self, k))
         return t
+
+    def set_mg_galerkin(self, enable: bool = True):
+        """Coarse levels of this context's multigrid preconditioner: Galerkin
+        P^T A P (True, the default) or rediscretised (False: cheaper
+        coarse-level passes, but more Krylov iterations at high density
+        contrast, and at some SIMP states the cycle stops helping and PCG
+        stalls)."""
+        self._call("vox8_set_mg_galerkin", int(bool(enable)))
+        self.mg_galerkin = bool(enable)
 
     def set_lobpcg_guard(self, guard: int):
         """Guard columns of the modal solve (0..2; -1 = the default): iterated

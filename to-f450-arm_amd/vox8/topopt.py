@@ -107,14 +107,17 @@ class SIMPOptimizer:
     (True: the modal solve overlaps the static one in a second context),
     device_tail (True: mode tracking, penalty, OC and bookkeeping in one device
     call with one host read; False: the same steps driven from the host),
-    modal_guard (guard columns of the modal solve: 0..2, or "auto": none until
-    an iteration's LOBPCG needs more than GUARD_TRIGGER iterations, then two;
-    late in a run the last tracked mode's neighbour comes close and the last
-    pair then converges slowly without them), effective_mass_direction (None,
+    modal_guard (guard columns of the modal solve: 0..2 (default 0), or "auto":
+    none until an iteration's LOBPCG needs more than GUARD_TRIGGER
+    iterations, then two; they help the last tracked pair when its neighbour
+    comes close, but with the Galerkin multigrid in the modal context each
+    guard column costs more than the iterations it saves), effective_mass_direction (None,
     or 0/1/2: after each modal solve the tracked modes' effective-mass
     fractions in that translation direction, sims/modal_participation.py:37-57,
     are computed on the device and the history gains "eff_mass" (per mode, in
     frequency order) and "mode_rank" (mode indices by decreasing fraction)).
+    The multigrid coarse levels follow ``fem.set_mg_galerkin`` in both
+    contexts.
     """
 
     GUARD_TRIGGER = 50
@@ -124,7 +127,7 @@ class SIMPOptimizer:
                  n_modes_track=4, mu_freq_init=0.10, mu_freq_max=50.0, mu_freq_ramp_every=5,
                  mu_freq_ramp_factor=1.5, F_list=None, solve_rtol=None, modal_tol=1e-4,
                  modal_maxiter=500, solve_maxiter=5000, concurrent_solves=True, device_tail=True,
-                 modal_guard="auto", effective_mass_direction=None):
+                 modal_guard=0, effective_mass_direction=None):
         self.geom, self.fem, self.load_cases = geom, fem, load_cases
         self.vol_frac = vol_frac
         self.r_min_vox = max(1.0, r_min_mm / (geom.dx * 1e3))
