@@ -215,6 +215,16 @@ int vox8_stats(vox8_ctx* ctx, int64_t n, const uint8_t* d_mask, const double* d_
 /* h_out[2k+1] = {X_j.v (k), X_j.X_j (k), v.v} for k column-major vectors. */
 int vox8_block_dots(vox8_ctx* ctx, int64_t n, int k, const double* d_X, int64_t ld,
                     const double* d_v, double* h_out);
+/* Allocation-free variant of vox8_block_dots.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (work
+ * recorded under HIP graph capture, tight serving loops).  No context; the
+ * same {X_j.v (k), X_j.X_j (k), v.v} go to DEVICE memory d_out (2k+1 doubles)
+ * on `hip_stream` (NULL = default stream) with no host synchronisation.  The
+ * reduction partials live in a fixed workspace of the library (one per GPU,
+ * 512 x 33 doubles, kept until the process ends), so k is limited to 1..16
+ * (larger k returns an error); calls that overlap on one GPU race. */
+int vox8_block_dots_noalloc(void* hip_stream, int64_t n, int k, const double* d_X, int64_t ld,
+                            const double* d_v, double* d_out);
 /* y = a x + b y on d_mask (NULL = all). */
 int vox8_axpby(vox8_ctx* ctx, int64_t n, const uint8_t* d_mask, double a, const double* d_x,
                double b, double* d_y);

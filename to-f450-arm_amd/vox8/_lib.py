@@ -69,6 +69,7 @@ SIGNATURES = {
     "vox8_oc_update": [_vp, _vp, _vp, _vp, _f64, _f64, _f64, _vp, _pf64],
     "vox8_stats": [_vp, _i64, _vp, _vp, _vp, _pf64],
     "vox8_block_dots": [_vp, _i64, _i32, _vp, _i64, _vp, _pf64],
+    "vox8_block_dots_noalloc": [_vp, _i64, _i32, _vp, _i64, _vp, _vp],
     "vox8_axpby": [_vp, _i64, _vp, _f64, _vp, _f64, _vp],
     "vox8_compose_density": [_vp, _vp, _vp, _vp, _vp],
     "vox8_simp_tail": [_vp, C.POINTER(TailArgs), _pf64],
