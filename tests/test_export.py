@@ -2,10 +2,11 @@
 
 The deck is parsed back and pinned two ways.  Structure: node coordinates
 from the node index, connectivity against the geometry's own
-``element_nodes`` for every kept element (Abaqus C3D8 order), each element's
-material bin against the density it was binned from, E and density of every
-bin against the SIMP and Pedersen laws, tip-mass nodes and value, clamped
-nodes, the frequency step.  Physics: K and M assembled from nothing but the
+``element_nodes`` for every kept element (Abaqus C3D8 order), the kept body
+against scipy.ndimage's component labelling, the material classes (equal
+sizes, ordered by stiffness, class E and density the means of their members'
+SIMP and Pedersen values, total mass preserved), tip-mass nodes and value,
+clamped nodes, the frequency step.  Physics: K and M assembled from nothing but the
 parsed deck (oracle element matrices per material) give the natural
 frequencies of the in-house model of the same structure (oracle FEM3D with
 its SIMP interpolation and tip mass): to rounding for the solid deck of a
@@ -124,14 +125,22 @@ def test_deck_structure(tmp_path, mode):
     assert sorted(elem_mat) == list(range(1, len(kept) + 1))
     if mode == "simp":
         r = rho[kept]
-        edges = np.linspace(r.min(), r.max() + 1e-12, 41)
+        E_el, D_el = simp_E(r, MAT["E"]), simp_rho(r, MAT["rho"])
+        members = {}
         for k in range(1, len(kept) + 1):
-            m = elem_mat[k]
-            b = min(int(np.searchsorted(edges, r[k - 1], side="right")) - 1, 39)
-            c = 0.5 * (edges[b] + edges[b + 1])
-            assert m["E"] == pytest.approx(simp_E(c, MAT["E"]), rel=1e-6)
-            assert m["rho"] == pytest.approx(simp_rho(c, MAT["rho"]), rel=1e-6)
+            members.setdefault(id(elem_mat[k]), []).append(k - 1)
+        groups = sorted(members.values(), key=lambda g: E_el[g].min())
+        sizes = [len(g) for g in groups]
+        assert len(groups) == min(40, len(kept)) and max(sizes) - min(sizes) <= 1
+        for a, b in zip(groups, groups[1:]):
+            assert E_el[a].max() <= E_el[b].min()
+        for g in groups:
+            m = elem_mat[g[0] + 1]
+            assert m["E"] == pytest.approx(E_el[g].mean(), rel=1e-6)
+            assert m["rho"] == pytest.approx(D_el[g].mean(), rel=1e-6)
             assert m["nu"] == MAT["nu"]
+        mass = sum(elem_mat[k]["rho"] for k in range(1, len(kept) + 1))
+        assert mass == pytest.approx(D_el.sum(), rel=1e-6)
     else:
         assert all(m["E"] == pytest.approx(MAT["E"]) and m["rho"] == pytest.approx(MAT["rho"])
                    for m in elem_mat.values())
@@ -213,6 +222,30 @@ def test_simp_deck_frequencies_within_binning_error(tmp_path):
         f_deck = _deck_frequencies(parse_inp(info["path"]), MAT["nu"], 6)
         errs.append(np.max(np.abs(f_deck / f_in - 1)))
     assert errs[0] < 0.03 and errs[1] < 1e-3 and errs[1] < errs[0], errs
+
+
+def test_largest_body_and_tip_fallback(tmp_path):
+    """Two bodies of kept voxels (a larger clamped bar and a detached block):
+    the solid deck keeps the larger, face-connected one only; with the top
+    layer removed, the tip mass moves to the highest kept node of each
+    motor-mount column."""
+    from vox8.export import active_element_grid, write_inp
+    geom = TipBox(10, 4, 4, h=1e-3, hole_frac=0.0, seed=1)
+    ex, ey, ez = active_element_grid(geom)
+    rho = np.full(len(ex), 0.01)
+    rho[(ey <= 1) & (ez <= 2)] = 1.0                      # bar along x, three layers high
+    rho[(ey == 3) & (ez == 3) & (ex >= 6) & (ex <= 8)] = 1.0   # detached block (edge contact only)
+    info = write_inp(str(tmp_path / "s.inp"), geom, rho, "solid", _Mat, tip_mass_kg=2e-3)
+    assert info["elements"] == int(((ey <= 1) & (ez <= 2)).sum())
+    d = parse_inp(info["path"])
+    nxy = (geom.nx + 1) * (geom.ny + 1)
+    used = np.unique(np.concatenate([geom.element_nodes(ex[e], ey[e], ez[e])
+                                     for e in np.nonzero((ey <= 1) & (ez <= 2))[0]]))
+    back = {k + 1: nid for k, nid in enumerate(used)}
+    tip = sorted(back[v] for v in d["nsets"]["NTIP"])
+    mount_cols = np.unique(np.asarray(geom.motor_mount_top_nodes()) % nxy)
+    want = [n for n in used if n % nxy in mount_cols and n // nxy == 3]
+    assert tip == sorted(want) and len(tip) > 0
 
 
 def test_rejects_bad_input(tmp_path):

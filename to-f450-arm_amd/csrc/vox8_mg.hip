@@ -282,35 +282,48 @@ __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __res
 }
 
 // x = Ainv b for up to CN columns at once (blockIdx.y selects the column
-// group), so the inverse is read once per group.  One row per wavefront (its
-// lanes stride over the row: coalesced reads; the columns of b come from
-// cache), reduced across the wavefront; 4 rows per block, fixed summation
-// order.
+// group), so the inverse is read once per group.  A block stages the group's
+// columns of b in LDS (n <= COARSE_MAX_N) and computes 16 rows: each
+// wavefront owns 4 rows, its lanes stride over the n entries of those rows
+// (coalesced reads), then reduce across the wavefront; fixed summation order.
+constexpr int COARSE_MAX_N = 4096;
 template <int CN>
 __global__ void __launch_bounds__(256)
 k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float* __restrict__ b,
                   float* __restrict__ x, const int* __restrict__ skip) {
+  extern __shared__ float bs[];                      // [CN][n]
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= n) return;
+  const int row0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
   const int c0 = blockIdx.y * CN;
-  float s[CN];
+  for (int e = threadIdx.x; e < CN * n; e += blockDim.x) {
+    const int c = e / n, k = e - c * n;
+    bs[e] = c0 + c < ncols ? b[(int64_t)(c0 + c) * n + k] : 0.0f;
+  }
+  __syncthreads();
+  float s[4][CN];
 #pragma unroll
-  for (int c = 0; c < CN; ++c) s[c] = 0.0f;
-  const float* ar = Ainv + (int64_t)row * n;
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < CN; ++c) s[i][c] = 0.0f;
   for (int k = lane; k < n; k += 64) {
-    const float a = ar[k];
 #pragma unroll
-    for (int c = 0; c < CN; ++c) s[c] += a * (c0 + c < ncols ? b[(int64_t)(c0 + c) * n + k] : 0.0f);
+    for (int i = 0; i < 4; ++i) {
+      const float a = row0 + i < n ? Ainv[(int64_t)(row0 + i) * n + k] : 0.0f;
+#pragma unroll
+      for (int c = 0; c < CN; ++c) s[i][c] += a * bs[c * n + k];
+    }
   }
 #pragma unroll
-  for (int c = 0; c < CN; ++c) {
-    float v = s[c];
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int col = c0 + c;
-    if (lane == 0 && col < ncols && !(skip && skip[col])) x[(int64_t)col * n + row] = v;
-  }
+    for (int c = 0; c < CN; ++c) {
+      float v = s[i][c];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      const int col = c0 + c;
+      if (lane == 0 && row0 + i < n && col < ncols && !(skip && skip[col]))
+        x[(int64_t)col * n + row0 + i] = v;
+    }
 }
 
 __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b,
@@ -756,19 +769,23 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
-    const unsigned rb = (unsigned)((mg.n_coarse + 3) / 4);
-    if (ncols <= 4)
-      hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, 1), dim3(256), 0, st, mg.n_coarse, ncols,
-                         (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
+    const unsigned rb = (unsigned)((mg.n_coarse + 15) / 16);
+    // staged columns: at most 64 KB of LDS (4 columns above 2048 DOFs)
+    if (ncols <= 4 || mg.n_coarse > 2048)
+      hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, (ncols + 3) / 4), dim3(256),
+                         sizeof(float) * 4 * mg.n_coarse, st, mg.n_coarse, ncols, (const float*)mg.Ainv,
+                         (const float*)L.b, L.x, skip);
     else
-      hipLaunchKernelGGL(k_mg_coarse_apply<8>, dim3(rb, (ncols + 7) / 8), dim3(256), 0, st,
-                         mg.n_coarse, ncols, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
+      hipLaunchKernelGGL(k_mg_coarse_apply<8>, dim3(rb, (ncols + 7) / 8), dim3(256),
+                         sizeof(float) * 8 * mg.n_coarse, st, mg.n_coarse, ncols,
+                         (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
     VOX8_HIP(hipGetLastError());
     return OK;
   }
   const double theta = 0.5 * (L.lam_max + L.lam_max / mg.alpha);
   if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true, skip);   // no coarse grid below
-  VOX8_TRY(smooth(L, mg.nu, mg.alpha, ncols, st, true, skip));
+  const int nu = (l > 0 && mg.nu_coarse > 0) ? mg.nu_coarse : mg.nu;
+  VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st, true, skip));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
   const int g = (exact_below || L.g.n_dofs < mg.w_min_dofs) ? 1 : mg.gamma;
@@ -791,7 +808,7 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     ep.skip = skip;
     VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
   }
-  return smooth(L, mg.nu, mg.alpha, ncols, st, false, skip);
+  return smooth(L, nu, mg.alpha, ncols, st, false, skip);
 }
 
 static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,

@@ -1,29 +1,38 @@
-"""Abaqus / CalculiX input deck of the modal problem (sims/export_abaqus_inp.py:57-185).
+"""Abaqus / CalculiX input deck of the modal problem.
 
-The deck lets a solver outside this project reproduce the modal result: plain
-Abaqus keywords, so the same file runs in CalculiX and Abaqus/Standard.  Two
-models, as in the reference:
+The deck lets a solver outside this project reproduce the modal result
+(the reference writes the same kind of deck, sims/export_abaqus_inp.py:57-185,
+and compares its CalculiX frequencies with the in-house ones): plain Abaqus
+keywords, so the same file runs in CalculiX and Abaqus/Standard.  Two models:
 
 ``simp``   every active element with the SIMP stiffness E (e_min + (1 - e_min)
-           rho^p) and the Pedersen mass density of FEM3D baked into
-           per-element materials, binned into ``n_bins`` density classes;
+           rho^p) and the Pedersen mass density of FEM3D.  A deck with one
+           material per element is valid but slow for the external solvers, so
+           the elements are grouped into ``n_bins`` material classes of equal
+           size (at most one element apart) by rank of their stiffness, and a
+           class gets the mean stiffness and the mean mass density of its
+           members: the deck's total mass is the model's, and with as many
+           classes as elements the deck is the model itself;
 ``solid``  only elements with rho > 0.5, as solid material, restricted to the
-           largest face-connected body (a detached fragment would otherwise
-           contribute six rigid-body modes).
+           largest body connected through voxel faces (a detached fragment
+           would otherwise contribute six rigid-body modes).  The bodies are
+           the connected components of the face-adjacency graph of the kept
+           voxels.
 
-Deck layout (same keywords, numbering and order as the reference's): nodes
-referenced by a kept element, numbered 1.. in ascending node-index order,
-coordinates (ix dx, iy dy, iz dz); C3D8 elements numbered 1.. in active-element
-order with the voxel node order (bottom face counter-clockwise, then top),
-which is Abaqus' C3D8 order; one ELSET / MATERIAL / SOLID SECTION per
-non-empty bin; the tip mass split equally over the motor-mount nodes as MASS
-elements; the fixed nodes clamped in 1-3; a FREQUENCY step for ``n_modes``.
+Deck layout (the file format the reference's tools read): nodes referenced by
+a kept element, numbered 1.. in ascending node-index order, coordinates
+(ix dx, iy dy, iz dz); C3D8 elements numbered 1.. in active-element order
+with the voxel node order (bottom face counter-clockwise, then top), which is
+Abaqus' C3D8 order; one ELSET / MATERIAL / SOLID SECTION per class; the tip
+mass split equally over the motor-mount nodes as MASS elements (when a design
+has no material at the motor-mount nodes, over the highest material node in
+each of their (x, y) columns, the surface the motor would rest on); the fixed
+nodes clamped in 1-3; a FREQUENCY step for ``n_modes``.
 
-The connectivity and node numbering are vectorised index arithmetic on the
-voxel grid (the reference loops over elements in Python): the production arm
-(61,142 active elements) takes 0.3 s, a 440x64x48 grid (1.35 M elements,
-179 MB deck) 6 s.  Runs on the host only; it needs the geometry, the
-active-order densities and the material.
+Connectivity and node numbering are vectorised index arithmetic on the voxel
+grid: the production arm (61,142 active elements) takes 0.3 s, a 440x64x48
+grid (1.35 M elements, 179 MB deck) 6 s.  Runs on the host only; it needs the
+geometry, the active-order densities and the material.
 """
 from __future__ import annotations
 
@@ -68,33 +77,71 @@ def active_element_grid(geom):
 
 
 def _largest_body(geom, ex, ey, ez, keep):
-    from scipy import ndimage
-    g = np.zeros((geom.nx, geom.ny, geom.nz), dtype=bool)
-    g[ex[keep], ey[keep], ez[keep]] = True
-    lab, n = ndimage.label(g)
-    if n > 1:
-        main = int(np.argmax(np.bincount(lab.ravel())[1:])) + 1
-        keep = keep & (lab[ex, ey, ez] == main)
-    return keep
+    """keep restricted to the largest face-connected body of kept voxels
+    (ties: the body holding the lowest active index)."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    idx = np.nonzero(keep)[0]
+    if len(idx) < 2:
+        return keep
+    nx, ny = geom.nx, geom.ny
+    lin = ex[idx] + nx * (ey[idx] + ny * ez[idx])           # voxel index of each kept element
+    pos = np.full(geom.nx * geom.ny * geom.nz, -1, dtype=np.int64)
+    pos[lin] = np.arange(len(idx))
+    src, dst = [], []
+    for axis, step, lim in ((ex, 1, nx), (ey, nx, ny), (ez, nx * ny, geom.nz)):
+        ok = axis[idx] + 1 < lim                            # the +1 neighbour exists
+        nb = pos[lin[ok] + step]
+        has = nb >= 0
+        src.append(np.nonzero(ok)[0][has])
+        dst.append(nb[has])
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    g = coo_matrix((np.ones(len(src)), (src, dst)), shape=(len(idx), len(idx)))
+    n, label = connected_components(g, directed=False)
+    if n == 1:
+        return keep
+    size = np.bincount(label, minlength=n)
+    big = int(np.flatnonzero(size == size.max())[0])       # lowest label = lowest index on ties
+    out = np.zeros_like(keep)
+    out[idx[label == big]] = True
+    return out
 
 
 def _tip_nodes(geom, used):
-    """Motor-mount nodes among the used ones; when the top face is absent, the
-    highest used nodes within the motor-pad radius (sims/export_abaqus_inp.py:
-    104-127).  Empty for a geometry without a motor mount."""
+    """Tip-mass nodes: the geometry's motor-mount nodes that the design keeps;
+    if it keeps none, the highest kept node in each (x, y) column of the
+    motor-mount nodes, where the motor would bear on the material.  Empty for
+    a geometry without a motor mount."""
     if not hasattr(geom, "motor_mount_top_nodes"):
         return np.array([], dtype=np.int64)
-    tip = np.intersect1d(np.asarray(geom.motor_mount_top_nodes(), dtype=np.int64), used)
+    mount = np.asarray(geom.motor_mount_top_nodes(), dtype=np.int64)
+    tip = np.intersect1d(mount, used)
     if len(tip):
         return tip
     nxy = (geom.nx + 1) * (geom.ny + 1)
-    ix, iy = used % (geom.nx + 1), (used % nxy) // (geom.nx + 1)
-    r = np.hypot(ix * geom.dx - geom.motor_centre[0], iy * geom.dy - geom.motor_centre[1])
-    near = used[r <= geom.motor_dia / 2.0 + geom.motor_keep_ring]
-    if len(near) == 0:
-        raise ValueError("no nodes near the motor mount; cannot place the tip mass")
-    zt = near // nxy
-    return near[zt >= zt.max() - 1]
+    columns = np.unique(mount % nxy)                        # (x, y) columns of the mount
+    col_of_used = used % nxy
+    sel = np.isin(col_of_used, columns)
+    cand, ccol = used[sel], col_of_used[sel]
+    if len(cand) == 0:
+        raise ValueError("the design keeps no material under the motor mount; cannot place the "
+                         "tip mass")
+    top = np.zeros(nxy, dtype=np.int64) - 1
+    np.maximum.at(top, ccol, cand // nxy)                   # highest layer per column
+    return np.sort(cand[cand // nxy == top[ccol]])
+
+
+def _classes(E_el, D_el, n_bins):
+    """Equal-size material classes by stiffness rank: (class of each element,
+    class E = mean member E, class density = mean member density)."""
+    n = len(E_el)
+    k = max(1, min(int(n_bins), n))
+    order = np.argsort(E_el, kind="stable")
+    cls = np.empty(n, dtype=np.int64)
+    cls[order] = (np.arange(n) * k) // n                    # sizes differ by at most one
+    cnt = np.bincount(cls, minlength=k)
+    return (cls, np.bincount(cls, weights=E_el, minlength=k) / cnt,
+            np.bincount(cls, weights=D_el, minlength=k) / cnt)
 
 
 def _lines16(ids):
@@ -138,10 +185,7 @@ def write_inp(path, geom, rho_active, mode, mat, tip_mass_kg=0.06, n_modes=6, n_
         bin_E, bin_D = np.array([mat.E]), np.array([mat.rho])
     else:
         r = rho_active[e_idx]
-        edges = np.linspace(r.min(), r.max() + 1e-12, n_bins + 1)
-        bins = np.clip(np.digitize(r, edges) - 1, 0, n_bins - 1)
-        centres = 0.5 * (edges[:-1] + edges[1:])
-        bin_E, bin_D = simp_E(centres, mat.E), simp_rho(centres, mat.rho)
+        bins, bin_E, bin_D = _classes(simp_E(r, mat.E), simp_rho(r, mat.rho), n_bins)
 
     out = [f"** F450 quadcopter arm - modal verification deck ({mode})\n",
            f"** {len(used)} nodes, {len(e_idx)} C3D8 elements, {3 * len(used)} DoF\n",
