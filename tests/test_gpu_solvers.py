@@ -95,6 +95,46 @@ def test_pcg_maxiter_reports_info(gpu):
     assert info["info"] == [3] and info["iters"] == [3]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_lobpcg_maxiter_reports_info(gpu, prec):
+    """LOBPCG stopped by maxiter: info = iterations done (scipy's convention,
+    src/fem.py:450-454 then falls back to eigsh; vox8 warns and returns the
+    last Ritz pairs, which are still M-orthonormal and above the true
+    eigenvalues)."""
+    import warnings
+    import scipy.linalg as sla
+    geom = MaskedBox(10, 4, 4, seed=8)
+    ofem, vfem = _pair(geom)
+    vfem.precision = prec
+    rho = random_density(ofem.n_active, 2)
+    lam, X, info = vfem.modal_block(rho, 3, 0.0, [], tol=1e-12, maxiter=2)
+    assert info["info"] == 2 and info["iters"] == 2 and info["max_res"] > 1e-12
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        vfem.modal_analysis(rho, n_modes=3, tip_mass_kg=0.0, tol=1e-12, maxiter=2)
+    assert any(issubclass(x.category, RuntimeWarning) for x in w)
+    free = ofem.free_dofs
+    exact = sla.eigh(ofem.assemble_K(rho)[free][:, free].toarray(),
+                     ofem.assemble_M(rho)[free][:, free].toarray(), eigvals_only=True,
+                     subset_by_index=[0, 2])
+    assert np.all(np.sort(lam) >= exact * (1 - 1e-12))
+
+
+def test_non_finite_density_is_rejected(gpu):
+    """NaN or inf densities from the host are an error before any GPU work
+    (the reference's LOBPCG would fail and fall back to eigsh, which fails
+    as well)."""
+    geom = MaskedBox(6, 3, 3, seed=8)
+    ofem, vfem = _pair(geom)
+    rho = random_density(ofem.n_active, 2)
+    rho[3] = np.nan
+    with pytest.raises(ValueError):
+        vfem.modal_analysis(rho, n_modes=2, tip_mass_kg=0.0)
+    rho[3] = np.inf
+    with pytest.raises(ValueError):
+        vfem.solve_block(rho, _tip_load(geom)[None, :])
+
+
 @pytest.mark.parametrize("k", [1, 4, 6])
 def test_lobpcg_matches_dense_eigh(gpu, k):
     geom = MaskedBox(8, 3, 3, hole_frac=0.1, seed=21)

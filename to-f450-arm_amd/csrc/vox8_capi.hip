@@ -298,6 +298,7 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   if (const char* e = getenv("VOX8_MG_GALERKIN")) c->mg.galerkin = e[0] != '0';
   if (const char* e = getenv("VOX8_MG_NU")) c->mg.nu = std::max(1, std::min(16, atoi(e)));
   if (const char* e = getenv("VOX8_MG_NUC")) c->mg.nu_coarse = std::max(0, std::min(16, atoi(e)));
+  if (const char* e = getenv("VOX8_MG_NUC_L")) c->mg.nu_coarse_level = std::max(1, atoi(e));
   if (const char* e = getenv("VOX8_MG_ALPHA")) c->mg.alpha = std::max(1.5, atof(e));
   if (const char* e = getenv("VOX8_MG_MARGIN")) c->mg.lam_margin = std::max(1.0, atof(e));
   c->st = (hipStream_t)hip_stream;

@@ -51,7 +51,8 @@ struct MultiGrid {
   std::vector<MGLevel> lv;
   int ncols = 0;               // capacity of the work vectors
   int nu = 2;                  // Chebyshev smoothing steps before and after
-  int nu_coarse = 0;           // the same on the coarse levels (0: nu)
+  int nu_coarse = 0;           // the same on levels >= nu_coarse_level (0: nu)
+  int nu_coarse_level = 1;
   int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
   int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when
                                // gamma = 2: their kernels are launch-latency bound

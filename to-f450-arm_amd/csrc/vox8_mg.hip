@@ -784,7 +784,7 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   }
   const double theta = 0.5 * (L.lam_max + L.lam_max / mg.alpha);
   if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true, skip);   // no coarse grid below
-  const int nu = (l > 0 && mg.nu_coarse > 0) ? mg.nu_coarse : mg.nu;
+  const int nu = (l >= mg.nu_coarse_level && mg.nu_coarse > 0) ? mg.nu_coarse : mg.nu;
   VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st, true, skip));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;

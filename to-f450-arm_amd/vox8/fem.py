@@ -267,12 +267,15 @@ class FEM3D:
         """Upload densities and form the element factors on the device.
 
         Device tensors are always re-read (kernels may have written them in
-        place without torch noticing); a NumPy array equal to the last one set
-        is skipped, which saves the host-to-device copy.
+        place without torch noticing) and taken as given; a NumPy array is
+        checked for NaN/inf (ValueError), and one equal to the last one set is
+        skipped, which saves the host-to-device copy.
         """
         params = (self.SIMP_PENALTY, self.E_MIN_FRAC, self.MASS_THRESHOLD, self.MASS_PENALTY_Q)
         if not isinstance(rho_active, torch.Tensor):
             rho_active = np.ascontiguousarray(rho_active, dtype=np.float64)
+            if not np.all(np.isfinite(rho_active)):
+                raise ValueError("densities must be finite (NaN or inf given)")
             last = self._density_key
             if (last is not None and last[1] == params and last[0].shape == rho_active.shape
                     and np.array_equal(last[0], rho_active)):
