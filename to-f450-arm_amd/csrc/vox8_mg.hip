@@ -282,35 +282,35 @@ __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __res
 }
 
 // x = Ainv b for up to CN columns at once (blockIdx.y selects the column
-// group), so the inverse is read once per group.  A block stages the group's
-// columns of b in LDS (n <= COARSE_MAX_N) and computes 16 rows: each
-// wavefront owns 4 rows, its lanes stride over the n entries of those rows
-// (coalesced reads), then reduce across the wavefront; fixed summation order.
-constexpr int COARSE_MAX_N = 4096;
+// group), so the inverse is read once per group (groups of 4 columns: more
+// groups give more workgroups, which this latency-bound product needs more
+// than fewer reads of the inverse).  Each wavefront owns 4 rows;
+// its lanes stride over the n entries (coalesced reads of the rows and of the
+// group's columns of b, which stay in cache), then reduce across the
+// wavefront.  16 rows per block, fixed summation order.
 template <int CN>
 __global__ void __launch_bounds__(256)
 k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float* __restrict__ b,
                   float* __restrict__ x, const int* __restrict__ skip) {
-  extern __shared__ float bs[];                      // [CN][n]
   const int lane = threadIdx.x & 63;
   const int row0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
   const int c0 = blockIdx.y * CN;
-  for (int e = threadIdx.x; e < CN * n; e += blockDim.x) {
-    const int c = e / n, k = e - c * n;
-    bs[e] = c0 + c < ncols ? b[(int64_t)(c0 + c) * n + k] : 0.0f;
-  }
-  __syncthreads();
+  const int nc = ncols - c0 < CN ? ncols - c0 : CN;
   float s[4][CN];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int c = 0; c < CN; ++c) s[i][c] = 0.0f;
+#pragma unroll 4
   for (int k = lane; k < n; k += 64) {
+    float bk[CN];
+#pragma unroll
+    for (int c = 0; c < CN; ++c) bk[c] = c < nc ? b[(int64_t)(c0 + c) * n + k] : 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const float a = row0 + i < n ? Ainv[(int64_t)(row0 + i) * n + k] : 0.0f;
 #pragma unroll
-      for (int c = 0; c < CN; ++c) s[i][c] += a * bs[c * n + k];
+      for (int c = 0; c < CN; ++c) s[i][c] += a * bk[c];
     }
   }
 #pragma unroll
@@ -321,7 +321,7 @@ k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float*
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
       const int col = c0 + c;
-      if (lane == 0 && row0 + i < n && col < ncols && !(skip && skip[col]))
+      if (lane == 0 && row0 + i < n && c < nc && !(skip && skip[col]))
         x[(int64_t)col * n + row0 + i] = v;
     }
 }
@@ -770,15 +770,8 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
     const unsigned rb = (unsigned)((mg.n_coarse + 15) / 16);
-    // staged columns: at most 64 KB of LDS (4 columns above 2048 DOFs)
-    if (ncols <= 4 || mg.n_coarse > 2048)
-      hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, (ncols + 3) / 4), dim3(256),
-                         sizeof(float) * 4 * mg.n_coarse, st, mg.n_coarse, ncols, (const float*)mg.Ainv,
-                         (const float*)L.b, L.x, skip);
-    else
-      hipLaunchKernelGGL(k_mg_coarse_apply<8>, dim3(rb, (ncols + 7) / 8), dim3(256),
-                         sizeof(float) * 8 * mg.n_coarse, st, mg.n_coarse, ncols,
-                         (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
+    hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, (ncols + 3) / 4), dim3(256), 0, st,
+                       mg.n_coarse, ncols, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
     VOX8_HIP(hipGetLastError());
     return OK;
   }

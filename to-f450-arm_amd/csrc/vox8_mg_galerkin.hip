@@ -17,6 +17,8 @@
 
 namespace vox8 {
 
+struct F3 { float x, y, z; };      // one node's three components: one 12-byte access
+
 // weight of parent corner A for the local node a of child c (per axis: the
 // node sits at half-position p = child bit + node offset in {0, 1, 2}; the
 // corner at 2 q, q in {0, 1})
@@ -236,8 +238,8 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
         const int col = c0 + k < ncols ? c0 + k : ncols - 1;
-        const float* xc = x + (int64_t)col * ld + 3 * j;
-        const float x0 = xc[0] * s0, x1 = xc[1] * s1, x2 = xc[2] * s2;
+        const F3 xv = *reinterpret_cast<const F3*>(x + (int64_t)col * ld + 3 * j);
+        const float x0 = xv.x * s0, x1 = xv.y * s1, x2 = xv.z * s2;
         acc[k][0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
         acc[k][1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
         acc[k][2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
@@ -343,6 +345,7 @@ int gal_apply(const Grid& g, const float* S32, const uint8_t* fixed, const float
   if (ncols <= 1) gal_apply_nc<1>(g, S32, fixed, x, y, ncols, ep, st);
   else if (ncols <= 2) gal_apply_nc<2>(g, S32, fixed, x, y, ncols, ep, st);
   else if (ncols <= 4) gal_apply_nc<4>(g, S32, fixed, x, y, ncols, ep, st);
+  else if (ncols <= 6) gal_apply_nc<6>(g, S32, fixed, x, y, ncols, ep, st);
   else gal_apply_nc<8>(g, S32, fixed, x, y, ncols, ep, st);
   VOX8_HIP(hipGetLastError());
   return OK;
