@@ -227,6 +227,11 @@ class SIMPOptimizer:
                 self._fem_modal = fem.twin()
             self._pool = ThreadPoolExecutor(max_workers=1)
             self._overlap = False
+            # one pair of events for the whole run: the solves' hand-over
+            # creates and destroys nothing while another thread may hold an
+            # open graph capture
+            self._ev_ready = torch.cuda.Event()
+            self._ev_done = torch.cuda.Event()
         self.tip = _tip_nodes_of(geom) if self.modal else np.array([], dtype=np.int64)
         self.w_target = (2.0 * np.pi * self.min_freq_Hz) ** 2 if self.modal else None
         self.history = {k: [] for k in ("compliance", "mass", "change", "vol_frac", "omega_1_Hz",
@@ -253,7 +258,14 @@ class SIMPOptimizer:
                 mfem.set_lobpcg_guard(self._guard)
                 self._guard_set = (id(mfem), self._guard)
         if self.modal and self._fem_modal is not None:
-            ready = torch.cuda.Event()
+            # Stream hand-over without record_stream: the modal stream waits
+            # for everything this thread queued so far (the previous tail's
+            # reads of the modes included), and this thread's stream waits
+            # for the modal solve before reading its outputs.  Tensors of
+            # either stream are then safe to free from any thread, and a free
+            # never records an event on another thread's stream (which could
+            # land inside a graph capture that thread has open).
+            ready, done = self._ev_ready, self._ev_done
             ready.record()
             X0 = self.X
 
@@ -266,9 +278,8 @@ class SIMPOptimizer:
                                                       maxiter=self.modal_maxiter)
                     if self.eff_dir is not None:
                         res = res + (effective_mass_device(self._fem_modal, res[1], self.eff_dir)[0],)
-                    done = torch.cuda.Event()
                     done.record()
-                return res, done
+                return res
 
             if self._overlap:
                 modal_job = self._pool.submit(run_modal)
@@ -289,12 +300,10 @@ class SIMPOptimizer:
         if self.modal:
             em = None
             if modal_job is not None:
-                res, done = modal_job.result()
+                res = modal_job.result()
                 lam, X, linfo = res[:3]
                 em = res[3] if len(res) > 3 else None
-                cur = torch.cuda.current_stream()
-                cur.wait_event(done)
-                X.record_stream(cur)
+                torch.cuda.current_stream().wait_event(self._ev_done)
             else:
                 lam, X, linfo = fem.modal_block(rho_phys, self.n_modes_track, self.tip_mass_kg,
                                                 self.tip, X0=self.X, tol=self.modal_tol,
