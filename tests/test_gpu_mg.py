@@ -63,12 +63,17 @@ def test_vcycle_matches_reference(gpu, shape, hole, gamma, wmin):
 @pytest.mark.parametrize("shape,hole,gamma,coarse", [((24, 6, 5), 0.0, 2, 700), ((33, 9, 7), 0.1, 2, 700),
                                                     ((33, 9, 7), 0.1, 1, 2000), ((70, 18, 14), 0.05, 2, 700),
                                                     ((45, 11, 9), 0.1, 2, 2000)])
-def test_galerkin_cycle_matches_reference(gpu, shape, hole, gamma, coarse):
+@pytest.mark.parametrize("lane", [False, True])
+def test_galerkin_cycle_matches_reference(gpu, monkeypatch, shape, hole, gamma, coarse, lane):
     """Galerkin coarse levels (vox8_set_mg_galerkin): the device cycle against
     the NumPy restatement with A_{l+1} = P^T A_l P formed by sparse matrix
     products, on densities down to 0.01 (contrast 1e6) with holes and odd
-    sizes; FP32 bar 2e-5 relative, symmetric and positive."""
+    sizes; FP32 bar 2e-5 relative, symmetric and positive.  lane=True applies
+    every coarse level with the one-node-per-lane kernel (entry-major blocks),
+    which the library otherwise keeps for levels of 65536 nodes and more."""
     from mg_reference import VCycle
+    if lane:
+        monkeypatch.setenv("VOX8_MG_LANE_NODES", "1")
     geom = MaskedBox(*shape, hole_frac=hole, seed=3)
     from vox8.fem import FEM3D, Material
     fem = FEM3D(geom, Material(E=1.7e9, nu=0.4, rho=1010.0))

@@ -301,6 +301,8 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   if (const char* e = getenv("VOX8_MG_NUC_L")) c->mg.nu_coarse_level = std::max(1, atoi(e));
   if (const char* e = getenv("VOX8_MG_ALPHA")) c->mg.alpha = std::max(1.5, atof(e));
   if (const char* e = getenv("VOX8_MG_MARGIN")) c->mg.lam_margin = std::max(1.0, atof(e));
+  if (const char* e = getenv("VOX8_MG_VTOP")) c->mg.v_top = std::max(0, atoi(e));
+  if (const char* e = getenv("VOX8_MG_LANE_NODES")) c->mg.lane_nodes = std::max(1LL, atoll(e));
   c->st = (hipStream_t)hip_stream;
   int s = 0;
   auto fail = [&](int code) { vox8_destroy(c); *out = nullptr; return code; };

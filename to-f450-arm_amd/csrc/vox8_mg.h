@@ -29,7 +29,8 @@ struct MGLevel {
   double lam_max = 2.0;        // estimate of lambda_max(D^-1 K)
   float* pv = nullptr;         // Lanczos vector (the other one is d)
   double* E64 = nullptr;       // Galerkin levels: per-element 24x24 matrices (FP64)
-  float* S32 = nullptr;        //   and the FP32 node blocks the smoother applies
+  float* S32 = nullptr;        //   and the FP32 node blocks the smoother applies,
+  bool s_by_entry = false;     //   stored entry-major (a node per lane) or node-major
   // work vectors, ncols column-major blocks of n_dofs
   float *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d2 = nullptr;
 };
@@ -54,9 +55,12 @@ struct MultiGrid {
   int nu_coarse = 0;           // the same on levels >= nu_coarse_level (0: nu)
   int nu_coarse_level = 1;
   int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
+  int v_top = 0;               // the finest v_top levels take one correction (V) whatever gamma
   int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when
                                // gamma = 2: their kernels are launch-latency bound
   int coarse_dofs = 2000;      // stop coarsening at this many DOFs (dense inverse)
+  int64_t lane_nodes = 65536;  // Galerkin levels with at least this many nodes apply one node
+                               // per lane; smaller ones spread a node over 8 lanes
   double alpha = 20.0;         // smoothing interval [lam_max/alpha, lam_max]
   bool galerkin = true;        // coarse levels P^T A P (element matrices); false: rediscretised
   double lam_margin = 1.1;     // lambda_max = margin x the largest Lanczos Ritz value
@@ -92,12 +96,12 @@ struct MultiGrid {
 // level, else the fine level's element matrices Ef64).
 int gal_build_level(const Grid& gf, const Grid& gc, const double* fK0, const double* dKe,
                     const double* Ef64, const uint8_t* fixf, const uint8_t* fixc, double* Ec64,
-                    float* S32, hipStream_t st);
+                    float* S32, bool by_entry, hipStream_t st);
 int gal_jacobi(const Grid& g, const double* E64, const uint8_t* fixed, float* dinv, hipStream_t st);
 int gal_coarse_assemble(const Grid& g, const double* E64, const uint8_t* fixed, double* A,
                         hipStream_t st);
-int gal_apply(const Grid& g, const float* S32, const uint8_t* fixed, const float* x, float* y,
-              int ncols, const Epilogue<float>& ep, hipStream_t st);
+int gal_apply(const Grid& g, const float* S32, bool by_entry, const uint8_t* fixed, const float* x,
+              float* y, int ncols, const Epilogue<float>& ep, hipStream_t st);
 
 // Destroy every captured cycle (device state is unchanged).
 void mg_drop_graphs(MultiGrid& mg);

@@ -475,7 +475,7 @@ static double tridiag_max_eig(const double* a, const double* b, int m) {
 // ---------------------------------------------------------------------------
 static int level_apply(const MGLevel& L, const float* x, float* y, int ncols, hipStream_t st,
                        Epilogue<float> ep) {
-  if (L.S32) return gal_apply(L.g, L.S32, L.fixed, x, y, ncols, ep, st);
+  if (L.S32) return gal_apply(L.g, L.S32, L.s_by_entry, L.fixed, x, y, ncols, ep, st);
   return launch_apply_fast<float>(L.g, L.pattern, OP_K, L.c32, L.fK, nullptr, L.fixed, x,
                                   L.g.n_dofs, y, nullptr, L.g.n_dofs, ncols, st, DotFuse(), ep);
 }
@@ -515,6 +515,7 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
       if (mg.galerkin) {
         VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 576 * g.n_elem)); L.E64 = (double*)p;
         VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * 128 * g.n_nodes)); L.S32 = (float*)p;
+        L.s_by_entry = g.n_nodes >= mg.lane_nodes;
       }
     }
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * g.n_dofs)); L.dinv = (float*)p;
@@ -610,7 +611,7 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
     if (l > 0 && L.E64) {
       const MGLevel& F = mg.lv[l - 1];
       VOX8_TRY(gal_build_level(F.g, L.g, l == 1 ? fK0 : nullptr, dKe, l == 1 ? nullptr : F.E64,
-                               F.fixed, L.fixed, L.E64, L.S32, st));
+                               F.fixed, L.fixed, L.E64, L.S32, L.s_by_entry, st));
       VOX8_TRY(gal_jacobi(L.g, L.E64, L.fixed, L.dinv, st));
       continue;
     }
@@ -781,7 +782,7 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st, true, skip));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
-  const int g = (exact_below || L.g.n_dofs < mg.w_min_dofs) ? 1 : mg.gamma;
+  const int g = (exact_below || L.g.n_dofs < mg.w_min_dofs || l < mg.v_top) ? 1 : mg.gamma;
   for (int j = 0; j < g; ++j) {
     hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
                        L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b, skip);

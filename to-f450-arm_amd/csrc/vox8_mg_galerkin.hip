@@ -158,7 +158,8 @@ __constant__ int GAL_D[14][3] = {{0, 0, 0},
                                  {-1, 0, 1}, {0, 0, 1}, {1, 0, 1},
                                  {-1, 1, 1}, {0, 1, 1}, {1, 1, 1}};
 
-__global__ void k_gal_nodeblocks(Grid g, const double* __restrict__ E64, float* __restrict__ S) {
+__global__ void k_gal_nodeblocks(Grid g, const double* __restrict__ E64, float* __restrict__ S,
+                                 int64_t sn, int64_t se) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= g.n_nodes) return;
   const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
@@ -179,7 +180,7 @@ __global__ void k_gal_nodeblocks(Grid g, const double* __restrict__ E64, float* 
         for (int c = 0; c < 3; ++c) acc[3 * r + c] += Ee[(3 * a + r) * 24 + 3 * b + c];
     }
 #pragma unroll
-    for (int q = 0; q < 9; ++q) S[n * 128 + blk * 9 + q] = (float)acc[q];
+    for (int q = 0; q < 9; ++q) S[n * sn + (blk * 9 + q) * se] = (float)acc[q];
   }
 }
 
@@ -294,17 +295,119 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
   }
 }
 
+// The same product with one node per lane, for levels large enough to fill
+// the GPU that way: the blocks are stored entry-major (S[e * n_nodes + n],
+// e = blk * 9 + 3 r + c), so each block entry and each neighbour's vector is
+// one coalesced load of consecutive nodes, and the sum over the 27 products
+// stays in the lane (no butterfly).  One loop trip takes the forward block
+// at n + d and the backward one at n - d (a rolled loop: unrolled, the 27
+// products' loads would not fit the register file).  A neighbour outside
+// the grid contributes a zero block read at the node itself.
+template <int MODE, int NC, bool TRANS>
+__device__ __forceinline__ void gal_lane_block(const Grid& g, int64_t n, int ix, int iy, int iz,
+                                               int dx, int dy, int dz, int blk,
+                                               const float* __restrict__ S, int64_t nn,
+                                               const float* const (&xc)[NC], const Epilogue<float>& ep,
+                                               float (&acc)[NC][3]) {
+  const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+  const bool in = jx >= 0 && jy >= 0 && jz >= 0 && jx < g.NX && jy < g.NY && jz < g.NZ;
+  const int64_t j = in ? n + dx + (int64_t)g.NX * (dy + (int64_t)g.NY * dz) : n;
+  const float* bp = S + (TRANS ? j : n) + (int64_t)blk * 9 * nn;
+  float b[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const int qq = TRANS ? 3 * (q % 3) + q / 3 : q;
+    b[q] = in ? bp[qq * nn] : 0.0f;
+  }
+  float s0 = 1.0f, s1 = 1.0f, s2 = 1.0f;
+  if (MODE == EPI_CHEB0) {                                 // x holds b; the input is c0 D^-1 b
+    s0 = ep.c0 * ep.dinv[3 * j];
+    s1 = ep.c0 * ep.dinv[3 * j + 1];
+    s2 = ep.c0 * ep.dinv[3 * j + 2];
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const F3 xv = *reinterpret_cast<const F3*>(xc[k] + 3 * j);
+    const float x0 = xv.x * s0, x1 = xv.y * s1, x2 = xv.z * s2;
+    acc[k][0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
+    acc[k][1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
+    acc[k][2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
+  }
+}
+
+template <int MODE, int NC>
+__global__ void __launch_bounds__(256)
+k_gal_apply_lane(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
+                 const uint8_t* __restrict__ fixed, const float* __restrict__ x,
+                 float* __restrict__ y, Epilogue<float> ep) {
+  const int wg = (blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
+  const int64_t n = (int64_t)wg * 256 + threadIdx.x;
+  if (n >= g.n_nodes) return;
+  const int64_t ld = g.n_dofs, nn = g.n_nodes;
+  const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
+  const int c0 = blockIdx.y * NC;
+  const float c0f = ep.c0;
+  const float* xc[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) xc[k] = x + (int64_t)(c0 + k < ncols ? c0 + k : ncols - 1) * ld;
+  float acc[NC][3];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
+  gal_lane_block<MODE, NC, false>(g, n, ix, iy, iz, 0, 0, 0, 0, S, nn, xc, ep, acc);
+#pragma unroll 1
+  for (int blk = 1; blk < 14; ++blk) {
+    const int dx = GAL_D[blk][0], dy = GAL_D[blk][1], dz = GAL_D[blk][2];
+    gal_lane_block<MODE, NC, false>(g, n, ix, iy, iz, dx, dy, dz, blk, S, nn, xc, ep, acc);
+    gal_lane_block<MODE, NC, true>(g, n, ix, iy, iz, -dx, -dy, -dz, blk, S, nn, xc, ep, acc);
+  }
+  const uint8_t fxn = fixed[n];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const int col = c0 + k;
+    if (col >= ncols || (ep.skip && ep.skip[col])) continue;
+    const int64_t o = (int64_t)col * ld + 3 * n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const bool fx = (fxn >> c) & 1;
+      const float val = fx ? 0.0f : acc[k][c];
+      if (MODE == EPI_PLAIN) {
+        y[o + c] = val;
+      } else if (MODE == EPI_RESID) {
+        const float r = fx ? 0.0f : ep.b[o + c] - val;
+        ep.r[o + c] = r;
+        ep.dnew[o + c] = ep.c2 * ep.dinv[3 * n + c] * r;
+      } else if (MODE == EPI_CHEB) {
+        const float d = x[o + c];
+        ep.xs[o + c] += d;
+        const float r = ep.r[o + c] - val;
+        ep.r[o + c] = r;
+        ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+      } else {                                             // EPI_CHEB0 (x holds b)
+        const float bv = x[o + c];
+        const float d = c0f * ep.dinv[3 * n + c] * bv;
+        ep.xs[o + c] = d;
+        const float r = fx ? 0.0f : bv - val;
+        ep.r[o + c] = r;
+        ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+      }
+    }
+  }
+}
+
+
 int gal_build_level(const Grid& gf, const Grid& gc, const double* fK0, const double* dKe,
                     const double* Ef64, const uint8_t* fixf, const uint8_t* fixc, double* Ec64,
-                    float* S32, hipStream_t st) {
+                    float* S32, bool by_entry, hipStream_t st) {
   if (Ef64)
     hipLaunchKernelGGL(k_gal_build<false>, dim3((unsigned)gc.n_elem), dim3(256), 0, st, gf, gc,
                        (const double*)nullptr, (const double*)nullptr, Ef64, fixf, fixc, Ec64);
   else
     hipLaunchKernelGGL(k_gal_build<true>, dim3((unsigned)gc.n_elem), dim3(256), 0, st, gf, gc, fK0,
                        dKe, (const double*)nullptr, fixf, fixc, Ec64);
+  // node-major records (8 lanes per node) or entry-major arrays (a node per lane)
+  const int64_t sn = by_entry ? 1 : 128, se = by_entry ? gc.n_nodes : 1;
   hipLaunchKernelGGL(k_gal_nodeblocks, dim3((unsigned)((gc.n_nodes + 255) / 256)), dim3(256), 0, st, gc,
-                     (const double*)Ec64, S32);
+                     (const double*)Ec64, S32, sn, se);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -327,26 +430,36 @@ int gal_coarse_assemble(const Grid& g, const double* E64, const uint8_t* fixed, 
 }
 
 template <int NC>
-static void gal_apply_nc(const Grid& g, const float* S32, const uint8_t* fixed, const float* x,
-                         float* y, int ncols, const Epilogue<float>& ep, hipStream_t st) {
-  const int nwg = (int)((g.n_nodes + 31) / 32);       // 32 nodes per workgroup
+static void gal_apply_nc(const Grid& g, const float* S32, bool by_entry, const uint8_t* fixed,
+                         const float* x, float* y, int ncols, const Epilogue<float>& ep,
+                         hipStream_t st) {
+  const int npw = by_entry ? 256 : 32;                // nodes per workgroup
+  const int nwg = (int)((g.n_nodes + npw - 1) / npw);
   const int per_xcd = (nwg + 7) / 8;
   const dim3 grid((unsigned)(8 * per_xcd), (unsigned)((ncols + NC - 1) / NC)), blk(256);
-  switch (ep.mode) {
-    case EPI_RESID: hipLaunchKernelGGL((k_gal_apply<EPI_RESID, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
-    case EPI_CHEB: hipLaunchKernelGGL((k_gal_apply<EPI_CHEB, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
-    case EPI_CHEB0: hipLaunchKernelGGL((k_gal_apply<EPI_CHEB0, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
-    default: hipLaunchKernelGGL((k_gal_apply<EPI_PLAIN, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;
+#define VOX8_GAL_LAUNCH(KERN)                                                                      \
+  switch (ep.mode) {                                                                               \
+    case EPI_RESID: hipLaunchKernelGGL((KERN<EPI_RESID, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break; \
+    case EPI_CHEB: hipLaunchKernelGGL((KERN<EPI_CHEB, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;   \
+    case EPI_CHEB0: hipLaunchKernelGGL((KERN<EPI_CHEB0, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break; \
+    default: hipLaunchKernelGGL((KERN<EPI_PLAIN, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;        \
   }
+  static const int unr = getenv("VOX8_GAL_UNR") ? atoi(getenv("VOX8_GAL_UNR")) : 1;
+  if (by_entry) {
+    VOX8_GAL_LAUNCH(k_gal_apply_lane)
+  } else {
+    VOX8_GAL_LAUNCH(k_gal_apply)
+  }
+#undef VOX8_GAL_LAUNCH
 }
 
-int gal_apply(const Grid& g, const float* S32, const uint8_t* fixed, const float* x, float* y,
-              int ncols, const Epilogue<float>& ep, hipStream_t st) {
-  if (ncols <= 1) gal_apply_nc<1>(g, S32, fixed, x, y, ncols, ep, st);
-  else if (ncols <= 2) gal_apply_nc<2>(g, S32, fixed, x, y, ncols, ep, st);
-  else if (ncols <= 4) gal_apply_nc<4>(g, S32, fixed, x, y, ncols, ep, st);
-  else if (ncols <= 6) gal_apply_nc<6>(g, S32, fixed, x, y, ncols, ep, st);
-  else gal_apply_nc<8>(g, S32, fixed, x, y, ncols, ep, st);
+int gal_apply(const Grid& g, const float* S32, bool by_entry, const uint8_t* fixed, const float* x,
+              float* y, int ncols, const Epilogue<float>& ep, hipStream_t st) {
+  if (ncols <= 1) gal_apply_nc<1>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
+  else if (ncols <= 2) gal_apply_nc<2>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
+  else if (ncols <= 4) gal_apply_nc<4>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
+  else if (ncols <= 6) gal_apply_nc<6>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
+  else gal_apply_nc<8>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
