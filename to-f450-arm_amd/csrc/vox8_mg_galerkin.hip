@@ -301,65 +301,129 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
 // one coalesced load of consecutive nodes, and the sum over the 27 products
 // stays in the lane (no butterfly).  One loop trip takes the forward block
 // at n + d and the backward one at n - d (a rolled loop: unrolled, the 27
-// products' loads would not fit the register file).  A neighbour outside
-// the grid contributes a zero block read at the node itself.
-template <int MODE, int NC, bool TRANS>
-__device__ __forceinline__ void gal_lane_block(const Grid& g, int64_t n, int ix, int iy, int iz,
-                                               int dx, int dy, int dz, int blk,
-                                               const float* __restrict__ S, int64_t nn,
-                                               const float* const (&xc)[NC], const Epilogue<float>& ep,
-                                               float (&acc)[NC][3]) {
-  const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
-  const bool in = jx >= 0 && jy >= 0 && jz >= 0 && jx < g.NX && jy < g.NY && jz < g.NZ;
-  const int64_t j = in ? n + dx + (int64_t)g.NX * (dy + (int64_t)g.NY * dz) : n;
-  const float* bp = S + (TRANS ? j : n) + (int64_t)blk * 9 * nn;
-  float b[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) {
-    const int qq = TRANS ? 3 * (q % 3) + q / 3 : q;
-    b[q] = in ? bp[qq * nn] : 0.0f;
-  }
-  float s0 = 1.0f, s1 = 1.0f, s2 = 1.0f;
-  if (MODE == EPI_CHEB0) {                                 // x holds b; the input is c0 D^-1 b
-    s0 = ep.c0 * ep.dinv[3 * j];
-    s1 = ep.c0 * ep.dinv[3 * j + 1];
-    s2 = ep.c0 * ep.dinv[3 * j + 2];
-  }
-#pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    const F3 xv = *reinterpret_cast<const F3*>(xc[k] + 3 * j);
-    const float x0 = xv.x * s0, x1 = xv.y * s1, x2 = xv.z * s2;
-    acc[k][0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
-    acc[k][1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
-    acc[k][2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
-  }
+// products' loads would not fit the register file).  Loads go through
+// buffer resources: the per-lane byte offset is the node's, the uniform
+// part (block entry, column) is a scalar offset, and a neighbour outside the
+// grid reads at an offset past the buffer, which returns 0.  Byte offsets
+// are 32-bit (the launcher checks the arrays are below 2^31 bytes).
+using GalRsrc = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t GAL_OOB = 0x80000000u;
+__device__ __forceinline__ GalRsrc gal_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float gal_ld(GalRsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// Values of lane - 1 (wave_shr:1) and lane + 1 (wave_shl:1) of the
+// wavefront; lane 0, respectively lane 63, keeps `edge`.
+__device__ __forceinline__ float lane_prev_or(float edge, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                               __builtin_bit_cast(int, v), 0x138,
+                                                               0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_next_or(float edge, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, edge),
+                                                               __builtin_bit_cast(int, v), 0x130,
+                                                               0xf, 0xf, false));
+}
+
+// The 27 offsets d = (dx, dy, dz) in the order o = (dx + 1) + 3 (dy + 1) +
+// 9 (dz + 1) are GAL_D mirrored around the diagonal: block |o - 13|, read
+// transposed at the neighbour when o < 13.  A loop trip takes one (dy, dz)
+// row of three offsets: the lane loads the neighbour row's vector at its own
+// x position, and its x - 1 / x + 1 values come from the adjacent lanes
+// (consecutive lanes hold consecutive nodes); lanes 0 and 63 load the one
+// value that lies outside the wavefront.
 template <int MODE, int NC>
 __global__ void __launch_bounds__(256)
 k_gal_apply_lane(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
                  const uint8_t* __restrict__ fixed, const float* __restrict__ x,
                  float* __restrict__ y, Epilogue<float> ep) {
   const int wg = (blockIdx.x % 8) * wg_per_xcd + blockIdx.x / 8;
-  const int64_t n = (int64_t)wg * 256 + threadIdx.x;
-  if (n >= g.n_nodes) return;
-  const int64_t ld = g.n_dofs, nn = g.n_nodes;
+  const int64_t n_raw = (int64_t)wg * 256 + threadIdx.x;
+  const bool live = n_raw < g.n_nodes;                  // every lane takes part in the shifts
+  const int64_t n = live ? n_raw : g.n_nodes - 1;
+  const int64_t ld = g.n_dofs;
+  const uint32_t nn = (uint32_t)g.n_nodes, n32 = (uint32_t)n;
   const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
+  const int lane = threadIdx.x & 63;
   const int c0 = blockIdx.y * NC;
   const float c0f = ep.c0;
-  const float* xc[NC];
+  const GalRsrc rs = gal_rsrc(S, 4u * 126u * nn);
+  const GalRsrc rx = gal_rsrc(x, 4u * (uint32_t)(ncols * ld));
+  const GalRsrc rdi = gal_rsrc(ep.dinv, MODE == EPI_CHEB0 ? 12u * nn : 0u);
+  uint32_t xoff[NC];
 #pragma unroll
-  for (int k = 0; k < NC; ++k) xc[k] = x + (int64_t)(c0 + k < ncols ? c0 + k : ncols - 1) * ld;
+  for (int k = 0; k < NC; ++k) xoff[k] = 4u * (uint32_t)((c0 + k < ncols ? c0 + k : ncols - 1) * ld);
   float acc[NC][3];
 #pragma unroll
   for (int k = 0; k < NC; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
-  gal_lane_block<MODE, NC, false>(g, n, ix, iy, iz, 0, 0, 0, 0, S, nn, xc, ep, acc);
 #pragma unroll 1
-  for (int blk = 1; blk < 14; ++blk) {
-    const int dx = GAL_D[blk][0], dy = GAL_D[blk][1], dz = GAL_D[blk][2];
-    gal_lane_block<MODE, NC, false>(g, n, ix, iy, iz, dx, dy, dz, blk, S, nn, xc, ep, acc);
-    gal_lane_block<MODE, NC, true>(g, n, ix, iy, iz, -dx, -dy, -dz, blk, S, nn, xc, ep, acc);
+  for (int r = 0; r < 9; ++r) {
+    const int dy = r % 3 - 1, dz = r / 3 - 1;
+    const int jy = iy + dy, jz = iz + dz;
+    const bool row_in = jy >= 0 && jz >= 0 && jy < g.NY && jz < g.NZ;
+    const uint32_t jc = n32 + (uint32_t)(g.NX * (dy + g.NY * dz));    // (ix, jy, jz)
+    bool in[3];
+    in[0] = row_in && ix > 0;
+    in[1] = row_in;
+    in[2] = row_in && ix + 1 < g.NX;
+    // the blocks of the three offsets
+    float b[3][9];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      const int o = t + 3 * r;
+      const int blk = o < 13 ? 13 - o : o - 13;
+      const uint32_t j = jc + (uint32_t)(t - 1);
+      const uint32_t vs = in[t] ? 4u * (o < 13 ? j : n32) : GAL_OOB;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        // transposed read for o < 13 (blk and o are uniform: scalar offsets)
+        const uint32_t e0 = 4u * (uint32_t)(blk * 9 + q) * nn;
+        const uint32_t e1 = 4u * (uint32_t)(blk * 9 + 3 * (q % 3) + q / 3) * nn;
+        b[t][q] = gal_ld(rs, vs, o < 13 ? e1 : e0);
+      }
+    }
+    // the row's vector: own x position, then the lanes' neighbours; lane 0
+    // fetches x - 1 and lane 63 x + 1 themselves
+    const uint32_t vc = row_in ? 12u * jc : GAL_OOB;
+    const uint32_t ve = (lane == 0 && in[0]) ? 12u * (jc - 1u)
+                      : ((lane == 63 && in[2]) ? 12u * (jc + 1u) : GAL_OOB);
+    float s[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (MODE == EPI_CHEB0) {                                // x holds b; the input is c0 D^-1 b
+        const float dc = gal_ld(rdi, vc + 4u * c, 0u), de = gal_ld(rdi, ve + 4u * c, 0u);
+        s[1][c] = c0f * dc;
+        s[0][c] = c0f * lane_prev_or(de, dc);
+        s[2][c] = c0f * lane_next_or(de, dc);
+      } else {
+        s[0][c] = s[1][c] = s[2][c] = 1.0f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      float xv[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float xc = gal_ld(rx, vc + 4u * c, xoff[k]), xe = gal_ld(rx, ve + 4u * c, xoff[k]);
+        xv[1][c] = xc;
+        xv[0][c] = lane_prev_or(xe, xc);
+        xv[2][c] = lane_next_or(xe, xc);
+      }
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        const float x0 = in[t] ? xv[t][0] * s[t][0] : 0.0f;
+        const float x1 = in[t] ? xv[t][1] * s[t][1] : 0.0f;
+        const float x2 = in[t] ? xv[t][2] * s[t][2] : 0.0f;
+        acc[k][0] += b[t][0] * x0 + b[t][1] * x1 + b[t][2] * x2;
+        acc[k][1] += b[t][3] * x0 + b[t][4] * x1 + b[t][5] * x2;
+        acc[k][2] += b[t][6] * x0 + b[t][7] * x1 + b[t][8] * x2;
+      }
+    }
   }
+  if (!live) return;
   const uint8_t fxn = fixed[n];
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
@@ -455,6 +519,11 @@ static void gal_apply_nc(const Grid& g, const float* S32, bool by_entry, const u
 
 int gal_apply(const Grid& g, const float* S32, bool by_entry, const uint8_t* fixed, const float* x,
               float* y, int ncols, const Epilogue<float>& ep, hipStream_t st) {
+  // the one-node-per-lane kernel addresses its blocks and vectors with
+  // 32-bit byte offsets
+  VOX8_CHECK(!by_entry || ((int64_t)128 * 4 * g.n_nodes < ((int64_t)1 << 31) &&
+                           (int64_t)4 * ncols * g.n_dofs < ((int64_t)1 << 31)),
+             "Galerkin level too large for 32-bit offsets");
   if (ncols <= 1) gal_apply_nc<1>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else if (ncols <= 2) gal_apply_nc<2>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else if (ncols <= 4) gal_apply_nc<4>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
