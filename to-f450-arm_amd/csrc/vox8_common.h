@@ -72,6 +72,17 @@ struct Grid {
   int64_t n_elem;
 };
 
+// Node index -> (i, j, k) with 32-bit divisions (node counts are below 2^31,
+// which vox8_create checks); 64-bit divisions cost several times as much.
+#ifdef __HIPCC__
+__device__ __forceinline__ void node_ijk(const Grid& g, int64_t n, int& i, int& j, int& k) {
+  const uint32_t u = (uint32_t)n, t = u / (uint32_t)g.NX;
+  i = (int)(u - t * (uint32_t)g.NX);
+  k = (int)(t / (uint32_t)g.NY);
+  j = (int)(t - (uint32_t)k * (uint32_t)g.NY);
+}
+#endif
+
 inline Grid make_grid(int nx, int ny, int nz) {
   Grid g;
   g.nx = nx; g.ny = ny; g.nz = nz;

@@ -94,7 +94,8 @@ __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
   if (skip && skip[col]) return;
   rf += (int64_t)col * gf.n_dofs;
   bc += (int64_t)col * gc.n_dofs;
-  const int I = (int)(n % gc.NX), J = (int)((n / gc.NX) % gc.NY), K = (int)(n / ((int64_t)gc.NX * gc.NY));
+  int I, J, K;
+  node_ijk(gc, n, I, J, K);
   float s0 = 0.f, s1 = 0.f, s2 = 0.f;
 #pragma unroll
   for (int dz = -1; dz <= 1; ++dz) {
@@ -137,7 +138,8 @@ __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
   if (skip && skip[col]) return;
   xc += (int64_t)col * gc.n_dofs;
   xf += (int64_t)col * gf.n_dofs;
-  const int i = (int)(n % gf.NX), j = (int)((n / gf.NX) % gf.NY), k = (int)(n / ((int64_t)gf.NX * gf.NY));
+  int i, j, k;
+  node_ijk(gf, n, i, j, k);
   const int x0 = i >> 1, x1 = (i + 1) >> 1, y0 = j >> 1, y1 = (j + 1) >> 1, z0 = k >> 1, z1 = (k + 1) >> 1;
   const float wx = (x0 == x1) ? 1.0f : 0.5f, wy = (y0 == y1) ? 1.0f : 0.5f, wz = (z0 == z1) ? 1.0f : 0.5f;
   float s0 = 0.f, s1 = 0.f, s2 = 0.f;

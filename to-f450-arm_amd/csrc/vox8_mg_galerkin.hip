@@ -208,7 +208,8 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
   const bool live = n < g.n_nodes;                    // whole 8-lane groups share this
   const int64_t ld = g.n_dofs, nn = g.n_nodes;
   const int64_t nc = live ? n : 0;
-  const int ix = (int)(nc % g.NX), iy = (int)((nc / g.NX) % g.NY), iz = (int)(nc / ((int64_t)g.NX * g.NY));
+  int ix, iy, iz;
+  node_ijk(g, nc, ix, iy, iz);
   const float c0f = ep.c0;
   float acc[NC][3];
 #pragma unroll
@@ -346,7 +347,8 @@ k_gal_apply_lane(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
   const int64_t n = live ? n_raw : g.n_nodes - 1;
   const int64_t ld = g.n_dofs;
   const uint32_t nn = (uint32_t)g.n_nodes, n32 = (uint32_t)n;
-  const int ix = (int)(n % g.NX), iy = (int)((n / g.NX) % g.NY), iz = (int)(n / ((int64_t)g.NX * g.NY));
+  int ix, iy, iz;
+  node_ijk(g, n, ix, iy, iz);
   const int lane = threadIdx.x & 63;
   const int c0 = blockIdx.y * NC;
   const float c0f = ep.c0;
