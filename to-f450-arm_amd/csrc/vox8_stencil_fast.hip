@@ -290,7 +290,7 @@ __device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mo
 // EPI_CHEB0 the same step from xs = 0, r = b with x holding b: d = c0 D^-1 b
 //   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r.
 template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE, int TY>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !DO_M ? 6 : (TY == 4 ? 3 : 1))))
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !DO_M ? 6 : 1)))
 k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T> C,
              const T* __restrict__ fK,
              const T* __restrict__ fM, const uint8_t* __restrict__ fixed,
@@ -603,8 +603,7 @@ static int tiles_y(const Grid& g, int ty) { return (g.NY + ty - 2) / (ty - 1); }
 // need more registers than two 12-wavefront workgroups per CU allow.
 template <typename T>
 static int tile_rows(const Grid& g) {
-  static const int ty64 = env_int("VOX8_FP64_TY", 8);
-  if (sizeof(T) != 4) return ty64 == 4 ? 4 : 8;
+  if (sizeof(T) != 4) return 8;
   return tiles_y(g, 12) * 12 < tiles_y(g, 8) * 8 ? 12 : 8;
 }
 
@@ -802,11 +801,6 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     if constexpr (sizeof(T) == 4) {                                                              \
       if (ty == 12) {                                                                            \
         VOX8_LAUNCH_TY(PAT, K, M, F, MD, 12);                                                    \
-        break;                                                                                   \
-      }                                                                                          \
-    } else {                                                                                     \
-      if (ty == 4) {                                                                             \
-        VOX8_LAUNCH_TY(PAT, K, M, F, MD, 4);                                                     \
         break;                                                                                   \
       }                                                                                          \
     }                                                                                            \
