@@ -104,8 +104,18 @@ def test_c5_twelve_modes_ranked_by_z_mass(gpu):
     # sensitivities from the two paths' solutions
     assert rel(m["dc"], out["fp64"]["dc"]) < 1e-6
     assert rel(m["dlam"], out["fp64"]["dlam"]) < 1e-5
-    print("lam rel err", lam_err.max(), "MAC min", macs.min(), "comp", comp_err.max(),
-          "z fractions", np.round(frac, 4), "rank", rank)
+    f_mixed = np.sqrt(lam) / (2 * np.pi)
+    print("c5 per-output errors against the plain FP64 path:")
+    print("  compliance rel. error per load case", np.array2string(comp_err, precision=2))
+    print("  true residual per load case <= 1e-8 of ||F||")
+    print("  f (Hz)", np.array2string(f_mixed, precision=4))
+    print("  eigenvalue rel. error per mode", np.array2string(lam_err, precision=2))
+    print("  MAC per mode", np.array2string(macs, precision=10))
+    print("  residual per mode", np.array2string(res, precision=2))
+    print("  z fraction abs. error per mode", np.array2string(np.abs(frac - out["fp64"]["frac"]), precision=2))
+    print("  z fractions", np.round(frac, 4), "rank", rank)
+    print("  sensitivities rel. error: dc", rel(m["dc"], out["fp64"]["dc"]), "dlam1",
+          rel(m["dlam"], out["fp64"]["dlam"]))
     for o in out.values():
         o["fem"].close()
 
