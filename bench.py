@@ -51,7 +51,7 @@ HBM_PEAK_GBS = 8000.0
 DOMINANT = "cheb_fp32"
 
 
-def make_optimizer(wl_name, precision, modal_maxiter=2000, galerkin=True, modal_guard=0):
+def make_optimizer(wl_name, precision, modal_maxiter=2000, galerkin=True, modal_guard="auto"):
     from vox8.fem import FEM3D, Material
     from vox8.topopt import SIMPOptimizer
     from vox8.workloads import E_PA, NU, RHO_MAT, WORKLOADS, load_cases, seeded_density
@@ -269,7 +269,7 @@ def cpu_baseline(wl, geom, pcg_iters, lob_iters, basis, slab_nz=6, budget_s=25.0
             "seconds_per_iter": round(per_iter, 2)}
 
 
-def run_workload(name, precision, steps, warmup, snapshot=None, galerkin=True, modal_guard=0):
+def run_workload(name, precision, steps, warmup, snapshot=None, galerkin=True, modal_guard="auto"):
     import numpy as np
     wl, geom, fem, opt, F, x0 = make_optimizer(name, precision, galerkin=galerkin,
                                                modal_guard=modal_guard)
@@ -323,7 +323,8 @@ def trajectory(name, precision, n_iter, modal_maxiter):
     """n_iter SIMP iterations from the seeded density, each timed on its own
     (the first includes the launch-shape tuning of the multigrid path)."""
     import torch
-    wl, geom, fem, opt, F, x0 = make_optimizer(name, precision, modal_maxiter=modal_maxiter)
+    wl, geom, fem, opt, F, x0 = make_optimizer(name, precision, modal_maxiter=modal_maxiter,
+                                               modal_guard="auto" if precision == "mixed" else 0)
     secs = []
     for i in range(n_iter):
         secs.append(time_steps(opt, 0, 1))
@@ -426,8 +427,9 @@ def main():
     ap.add_argument("--plain-maxiter", type=int, default=2000,
                     help="LOBPCG iteration cap of the plain FP64 (Jacobi) comparison path")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--modal-guard", default="0", choices=["auto", "0", "1", "2"],
-                    help="LOBPCG guard columns of the SIMP loop's modal solve")
+    ap.add_argument("--modal-guard", default="auto", choices=["auto", "0", "1", "2"],
+                    help="LOBPCG guard columns of the SIMP loop's modal solve (auto: two or none, "
+                         "whichever costs less; SIMPOptimizer modal_guard)")
     ap.add_argument("--coarse", default="galerkin", choices=["galerkin", "rediscretised"],
                     help="coarse levels of the multigrid preconditioner")
     ap.add_argument("--no-kernels", action="store_true",
@@ -489,7 +491,9 @@ def main():
     h = opt.history
     for i in range(args.warmup, len(h["time_s"])):
         print(f"[bench] iteration {i + 1}: {h['time_s'][i] * 1e3:.1f} ms, pcg {h['pcg_iters'][i]}, "
-              f"lobpcg {h['lobpcg_iters'][i]}", file=sys.stderr, flush=True)
+              f"lobpcg {h['lobpcg_iters'][i]}"
+              + (f" (guard {h['lobpcg_guard'][i]})" if i < len(h["lobpcg_guard"]) else ""),
+              file=sys.stderr, flush=True)
     roofs = {} if args.no_kernels else kernel_rooflines(fem, geom, opt.rho_phys)
     # Each workload's contexts are released before the next one starts: every
     # stream alive in the process shares the few hardware queues HIP maps
@@ -526,7 +530,8 @@ def main():
     if not args.no_plain and args.precision != "fp64":
         # one iteration of the plain path from the multigrid path's state at
         # the start of the timed window (same design, same warm starts)
-        _, _, _, popt, _, _ = make_optimizer(args.workload, "fp64", modal_maxiter=args.plain_maxiter)
+        _, _, _, popt, _, _ = make_optimizer(args.workload, "fp64", modal_maxiter=args.plain_maxiter,
+                                             modal_guard=0)
         popt.load_state(snap, overlap=True)
         pdt = time_steps(popt, 0, 1)
         ph = popt.history
@@ -562,7 +567,7 @@ def main():
                    "n_dofs": geom.n_dofs, "pcg_rtol": wl.pcg_rtol, "lobpcg_tol": wl.modal_tol,
                    "mean_pcg_iters_per_lc": round(pcg_it, 1), "mean_lobpcg_iters": round(lob_it, 1),
                    "precision": args.precision,
-                   "multigrid_coarse_levels": args.coarse},
+                   "multigrid_coarse_levels": args.coarse, "lobpcg_guard": args.modal_guard},
         "roofline": roofs.get(DOMINANT),
         "kernels": {k: v for k, v in roofs.items() if k != DOMINANT},
         "solvers": solvers,

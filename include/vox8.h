@@ -108,6 +108,13 @@ int vox8_lobpcg(vox8_ctx* ctx, int k, double tol, int maxiter, int prec, const d
  * reuses the previous solve's guard columns.  -1 restores the default
  * (environment VOX8_LOBPCG_GUARD, else 0). */
 int vox8_set_lobpcg_guard(vox8_ctx* ctx, int guard);
+/* Ritz values of the last vox8_lobpcg solve's whole block, ascending: the k
+ * wanted eigenvalues followed by those of the guard columns (k + guard
+ * values; *n_out = 0 before the first solve).  Copies at most n_max into
+ * h_out.  No reference counterpart: it lets a caller see the gap between the
+ * last wanted eigenvalue and the next one, which decides whether guard
+ * columns pay (vox8.topopt.SIMPOptimizer's "auto" policy). */
+int vox8_lobpcg_ritz(vox8_ctx* ctx, double* h_out, int n_max, int* n_out);
 
 /* The geometric-multigrid preconditioner used by the MIXED solvers:
  * z = cycle(r) (W-cycle by default) for nrhs FP64 column-major vectors (ld = n_dofs); r and z

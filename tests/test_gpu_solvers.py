@@ -163,7 +163,9 @@ def test_lobpcg_matches_dense_eigh(gpu, k):
 def test_lobpcg_guard_columns_same_modes(gpu, guard, prec):
     """Guard columns (vox8_set_lobpcg_guard) are iterated but not returned:
     the k wanted pairs still match dense eigh, cold and warm started, and -1
-    restores the default."""
+    restores the default.  vox8_lobpcg_ritz reports the whole block: k + guard
+    ascending values, the first k the returned eigenvalues, the guard ones
+    upper bounds of the next eigenvalues (Rayleigh-Ritz)."""
     from vox8.fem import FEM3D, Material
     geom = MaskedBox(8, 3, 3, hole_frac=0.1, seed=21)
     ofem, _ = _pair(geom)
@@ -173,8 +175,10 @@ def test_lobpcg_guard_columns_same_modes(gpu, guard, prec):
     Kff = ofem.assemble_K(rho)[free][:, free].toarray()
     Mff = ofem.assemble_M(rho)[free][:, free].toarray()
     k = 4
+    lam_all = sla.eigh(Kff, Mff, eigvals_only=True, subset_by_index=[0, k + guard - 1])
     lam_ref, V = sla.eigh(Kff, Mff, subset_by_index=[0, k - 1])
     tol = 1e-9 * 1.7e9 * geom.dx
+    assert vfem.lobpcg_ritz().size == 0
     vfem.set_lobpcg_guard(guard)
     X0 = None
     for _ in range(2):                      # cold, then warm (reusing the guard columns)
@@ -186,6 +190,10 @@ def test_lobpcg_guard_columns_same_modes(gpu, guard, prec):
         Xn = X.cpu().numpy()[order]
         for j in range(k):
             assert mac(Xn[j][free], V[:, j]) > 0.9999
+        ritz = vfem.lobpcg_ritz()
+        assert ritz.size == k + guard and np.all(np.diff(ritz) >= 0)
+        assert np.allclose(ritz[:k], lam[order], rtol=1e-12, atol=0)
+        assert np.all(ritz[k:] >= lam_all[k:] * (1 - 1e-9))
         X0 = X
     vfem.set_lobpcg_guard(-1)
     with pytest.raises(Exception):

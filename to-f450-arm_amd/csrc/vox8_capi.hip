@@ -640,6 +640,14 @@ int vox8_set_lobpcg_guard(vox8_ctx* c, int guard) {
   return OK;
 }
 
+int vox8_lobpcg_ritz(vox8_ctx* c, double* h_out, int n_max, int* n_out) {
+  VOX8_CHECK(c != nullptr && n_out != nullptr && (h_out != nullptr || n_max == 0), "null argument");
+  const int n = c->lw.ritz_n;
+  for (int j = 0; j < n && j < n_max; ++j) h_out[j] = c->lw.ritz[j];
+  *n_out = n;
+  return OK;
+}
+
 int vox8_sensitivities(vox8_ctx* c, const double* d_U, int nrhs, int64_t ld, const double* h_w,
                        const double* d_phi, double lam, const double* d_rho, double p, double e_min,
                        double mthr, double mq, double* d_ce, double* d_dc, double* d_dlam,

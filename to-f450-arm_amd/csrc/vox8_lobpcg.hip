@@ -680,6 +680,8 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
     VOX8_HIP(hipGetLastError());
     w.guard_cols = guard;
   }
+  for (int j = 0; j < k; ++j) w.ritz[j] = w.host[j];
+  w.ritz_n = k;
   double mx = 0.0;
   for (int j = 0; j < kw; ++j) {
     lam_out[j] = w.host[j];

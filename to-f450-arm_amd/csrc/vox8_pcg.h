@@ -78,6 +78,8 @@ struct LobpcgWork {
   double* guard = nullptr;    // last solve's guard vectors [gmax][n] (warm start of the next)
   int guard_cols = 0;         // columns held in `guard` (0 = none)
   int guard_k = 0;            // wanted pairs of the solve that left them
+  double ritz[12] = {};       // Ritz values of the last solve's block (wanted + guard), ascending
+  int ritz_n = 0;
   int kmax = 0;
   int64_t n = 0;
   int gram_chunks = 0;

@@ -440,6 +440,15 @@ class FEM3D:
         when the next eigenvalue is close to it."""
         self._call("vox8_set_lobpcg_guard", int(guard))
 
+    def lobpcg_ritz(self):
+        """Ritz values of the last modal solve's whole block, ascending: the
+        wanted eigenvalues, then the guard columns' (numpy; empty before the
+        first solve)."""
+        out = (C.c_double * 12)()
+        n = C.c_int()
+        self._call("vox8_lobpcg_ritz", out, 12, C.byref(n))
+        return np.array(out[:n.value])
+
     def modal_block(self, rho_active, n_modes=6, tip_mass_kg=0.060, tip_mass_nodes=None,
                     X0=None, tol=1e-4, maxiter=500, seed=0, precision=None):
         """Device-resident modal solve: returns (lam (k,) numpy, X (k, n_dofs) torch, info)."""

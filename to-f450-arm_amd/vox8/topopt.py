@@ -108,10 +108,17 @@ class SIMPOptimizer:
     device_tail (True: mode tracking, penalty, OC and bookkeeping in one device
     call with one host read; False: the same steps driven from the host),
     modal_guard (guard columns of the modal solve: 0..2 (default 0), or "auto":
-    none until an iteration's LOBPCG needs more than GUARD_TRIGGER
-    iterations, then two; they help the last tracked pair when its neighbour
-    comes close, but with the Galerkin multigrid in the modal context each
-    guard column costs more than the iterations it saves), effective_mass_direction (None,
+    two guard columns or none, whichever has cost less per solve (LOBPCG
+    iterations x block width).  The first solve runs without them and the
+    second with; afterwards the setting not in use is tried for one solve,
+    every 2 iterations at first and up to every GUARD_RETRY while the trials
+    confirm the choice.  A trial is compared with the solve just before it
+    (nearly the same design), and the choice follows the median of the last
+    GUARD_SAMPLES such cost ratios.  They pay
+    when the last tracked eigenvalues have close neighbours or the block is
+    too small for the cluster it tracks, and cost a third more per iteration
+    otherwise),
+    effective_mass_direction (None,
     or 0/1/2: after each modal solve the tracked modes' effective-mass
     fractions in that translation direction, sims/modal_participation.py:37-57,
     are computed on the device and the history gains "eff_mass" (per mode, in
@@ -120,7 +127,8 @@ class SIMPOptimizer:
     contexts.
     """
 
-    GUARD_TRIGGER = 50
+    GUARD_RETRY = 16
+    GUARD_SAMPLES = 5
 
     def __init__(self, geom, fem: FEM3D, load_cases, vol_frac=0.30, r_min_mm=2.0, move=0.2,
                  max_iter=100, tol_change=0.01, weights=None, min_freq_Hz=None, tip_mass_kg=0.060,
@@ -150,6 +158,12 @@ class SIMPOptimizer:
         self.eff_dir = effective_mass_direction
         self._guard = 0 if modal_guard == "auto" else int(modal_guard)
         self._guard_set = None
+        self._guard_prev = None        # "auto": (guard, cost) of the last modal solve
+        self._guard_probing = False    # "auto": the next solve tries the other setting
+        self._guard_next_probe = 1     # "auto": iteration after which to try it
+        self._guard_interval = 2       # "auto": iterations between trials
+        self._guard_ratios = []        # "auto": log cost ratios (guard / none) of recent trials
+        self._widths = set()           # block widths the modal solve has run with
         self._fem_modal = None
         if F_list is None:
             F_list = []
@@ -236,7 +250,7 @@ class SIMPOptimizer:
         self.w_target = (2.0 * np.pi * self.min_freq_Hz) ** 2 if self.modal else None
         self.history = {k: [] for k in ("compliance", "mass", "change", "vol_frac", "omega_1_Hz",
                                         "mu_freq", "freq_violation", "pcg_iters", "lobpcg_iters",
-                                        "time_s", "eff_mass", "mode_rank")}
+                                        "time_s", "eff_mass", "mode_rank", "lobpcg_guard")}
         self.it = 0
         return self
 
@@ -311,11 +325,9 @@ class SIMPOptimizer:
                 if self.eff_dir is not None:
                     em = effective_mass_device(fem, X, self.eff_dir)[0]
             lob_iters = linfo["iters"]
-            if self.modal_guard == "auto" and lob_iters > self.GUARD_TRIGGER and self._guard != 2:
-                self._guard = 2
-                # the next modal solve is two columns wider: run it alone so
-                # the launch shapes of that width are tuned on an idle GPU
-                self._overlap = False
+            self.history["lobpcg_guard"].append(self._guard)
+            if self.modal_guard == "auto":
+                self._auto_guard(lob_iters, it)
             order = np.argsort(lam)
             lam = np.maximum(lam[order], 0.0)
             if not np.array_equal(order, np.arange(len(order))):
@@ -341,6 +353,41 @@ class SIMPOptimizer:
         h["lobpcg_iters"].append(lob_iters)
         h["time_s"].append(time.perf_counter() - t0)
         return change
+
+    def _auto_guard(self, lob_iters, it):
+        """The "auto" guard policy (class docstring): after a trial of the
+        other setting, record the cost ratio of the two adjacent solves and
+        choose the setting the recorded ratios favour."""
+        g, k = self._guard, self.n_modes_track
+        if k + 2 > 12:
+            return                                  # no room for guard columns
+        cost = lob_iters * (k + g) / k
+        prev = self._guard_prev
+        self._guard_prev = (g, cost)
+        self._widths.add(k + g)
+        nxt = g
+        if self._guard_probing:
+            # the trial ran right after a solve with the other setting:
+            # adjacent designs, so the two iteration counts compare
+            self._guard_probing = False
+            c2, c0 = (cost, prev[1]) if g == 2 else (prev[1], cost)
+            self._guard_ratios.append(np.log(c2 / c0))
+            del self._guard_ratios[:-self.GUARD_SAMPLES]
+            # the median of the recent ratios: one trial next to a slow
+            # solve (a mode swap) does not flip the choice
+            nxt = 2 if np.median(self._guard_ratios) < 0 else 0
+            same = nxt == prev[0]
+            self._guard_interval = min(2 * self._guard_interval, self.GUARD_RETRY) if same else 2
+            self._guard_next_probe = it + self._guard_interval
+        elif it >= self._guard_next_probe:
+            nxt = 2 - g
+            self._guard_probing = True
+        if nxt != g:
+            self._guard = nxt
+            if k + nxt not in self._widths:
+                # a block width not run before: that solve runs alone, so the
+                # launch shapes of the width are tuned on an idle GPU
+                self._overlap = False
 
     def close(self, fem: bool = False):
         """Release the modal context, its worker thread and stream, in that
