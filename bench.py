@@ -187,6 +187,33 @@ def kernel_rooflines(fem, geom, rho, nrhs=6, reps=30):
     return res
 
 
+def summarize_trace(path, top=40, name_filter=""):
+    """Per-(kernel, grid) table of a rocprofv3 kernel trace (CSV): rocprofv3's
+    --stats groups by kernel name, which merges a kernel's launches on every
+    multigrid level and at every tuning candidate; split by grid size, one
+    kernel at one problem size can be compared with the HIP-event timings."""
+    import csv
+    from collections import defaultdict
+    agg = defaultdict(list)
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"]
+            if name_filter and name_filter not in name:
+                continue
+            grid = r.get("Grid_Size") or r.get("Grid_Size_X") or ""
+            agg[(name, grid)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total = sum(sum(v) for v in agg.values())
+    rows = sorted(agg.items(), key=lambda kv: -sum(kv[1]))
+    out = [f"Total GPU kernel time: {total / 1e3:.1f} ms over {sum(len(v) for v in agg.values())} launches.",
+           "", "| total ms | calls | mean us | min us | max us | grid | kernel |",
+           "|---:|---:|---:|---:|---:|---:|---|"]
+    for (name, grid), t in rows[:top]:
+        short = name.split("(")[0].replace("|", "\\|")
+        out.append(f"| {sum(t) / 1e3:.2f} | {len(t)} | {sum(t) / len(t):.1f} | {min(t):.1f} | {max(t):.1f} "
+                   f"| {grid} | `{short}` |")
+    return "\n".join(out) + "\n"
+
+
 def load_traffic(name, geom):
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if it
     covers this kernel at this grid."""
@@ -440,6 +467,11 @@ def main():
                     help="instead of the bench line: N SIMP iterations of the workload on the "
                          "multigrid path and on the plain FP64 path from the same seeded "
                          "density, one JSON line per path with per-iteration times")
+    ap.add_argument("--summarize-trace", default=None, metavar="KERNEL_TRACE_CSV",
+                    help="print a per-(kernel, grid) table of a rocprofv3 kernel trace and exit "
+                         "(with --top N and --trace-filter SUBSTRING)")
+    ap.add_argument("--top", type=int, default=40)
+    ap.add_argument("--trace-filter", default="")
     ap.add_argument("--c4-only", action="store_true",
                     help="only the batched-designs measurement, as one JSON object (the main "
                          "bench runs it this way, in a child process)")
@@ -447,6 +479,9 @@ def main():
                     help="only the per-kernel timings (and a 1 GiB device copy as a counter "
                          "calibration), for rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace, args.top, args.trace_filter), end="")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
