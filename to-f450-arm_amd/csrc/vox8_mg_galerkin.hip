@@ -19,6 +19,19 @@ namespace vox8 {
 
 struct F3 { float x, y, z; };      // one node's three components: one 12-byte access
 
+// Buffer-resource loads: the per-lane byte offset is the node's, a uniform
+// part (block entry, column) can go in the scalar offset, and an offset past
+// the buffer (GAL_OOB) returns 0.  Byte offsets are 32-bit (the launcher
+// checks the arrays are below 2^31 bytes).
+using GalRsrc = __amdgpu_buffer_rsrc_t;
+constexpr uint32_t GAL_OOB = 0x80000000u;
+__device__ __forceinline__ GalRsrc gal_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float gal_ld(GalRsrc r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
 // weight of parent corner A for the local node a of child c (per axis: the
 // node sits at half-position p = child bit + node offset in {0, 1, 2}; the
 // corner at 2 q, q in {0, 1})
@@ -214,38 +227,55 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
   float acc[NC][3];
 #pragma unroll
   for (int k = 0; k < NC; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
-  if (live) {
-    for (int t = l8; t < 27; t += 8) {
-      const int blk = t < 14 ? t : t - 13;
-      const int sgn = t < 14 ? 1 : -1;
-      const int jx = ix + sgn * GAL_D[blk][0], jy = iy + sgn * GAL_D[blk][1], jz = iz + sgn * GAL_D[blk][2];
-      if (jx < 0 || jy < 0 || jz < 0 || jx >= g.NX || jy >= g.NY || jz >= g.NZ) continue;
-      const int64_t j = jx + (int64_t)g.NX * (jy + (int64_t)g.NY * jz);
-      const float* bp = S + (t < 14 ? n : j) * 128 + blk * 9;
-      float b[9];
+  // Items t = l8 + 8 i (i < 4, t < 27), unrolled so every load of the lane is
+  // issued before the first is used.  Item t is the offset o of the 27 in the
+  // order (dx + 1) + 3 (dy + 1) + 9 (dz + 1): forward block t at o = t + 13,
+  // backward block t - 13 at o = 26 - t (GAL_D lists the forward offsets in
+  // that order).  Loads go through buffer resources; an item past 26, a
+  // neighbour outside the grid or a node past the end reads past the buffer,
+  // which returns 0.
+  const GalRsrc rs = gal_rsrc(S, 4u * 128u * (uint32_t)nn);
+  const GalRsrc rx = gal_rsrc(x, 4u * (uint32_t)(ncols * ld));
+  const GalRsrc rdi = gal_rsrc(ep.dinv, MODE == EPI_CHEB0 ? 12u * (uint32_t)nn : 0u);
+  uint32_t xoff[NC];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) b[q] = bp[q];
-      if (t >= 14) {                                   // backward: the transpose
-        float u;
-        u = b[1]; b[1] = b[3]; b[3] = u;
-        u = b[2]; b[2] = b[6]; b[6] = u;
-        u = b[5]; b[5] = b[7]; b[7] = u;
-      }
-      float s0 = 1.0f, s1 = 1.0f, s2 = 1.0f;
-      if (MODE == EPI_CHEB0) {                         // x holds b; the input is c0 D^-1 b
-        s0 = c0f * ep.dinv[3 * j];
-        s1 = c0f * ep.dinv[3 * j + 1];
-        s2 = c0f * ep.dinv[3 * j + 2];
-      }
+  for (int k = 0; k < NC; ++k) xoff[k] = 4u * (uint32_t)((c0 + k < ncols ? c0 + k : ncols - 1) * ld);
 #pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        const int col = c0 + k < ncols ? c0 + k : ncols - 1;
-        const F3 xv = *reinterpret_cast<const F3*>(x + (int64_t)col * ld + 3 * j);
-        const float x0 = xv.x * s0, x1 = xv.y * s1, x2 = xv.z * s2;
-        acc[k][0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
-        acc[k][1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
-        acc[k][2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
-      }
+  for (int i = 0; i < 4; ++i) {
+    const int t = l8 + 8 * i;
+    const bool fwd = t < 14;
+    const int o = fwd ? t + 13 : 26 - t;
+    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+    const int blk = fwd ? t : t - 13;
+    const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+    const bool in = live && t < 27 && jx >= 0 && jy >= 0 && jz >= 0 && jx < g.NX && jy < g.NY &&
+                    jz < g.NZ;
+    const uint32_t j = (uint32_t)(jx + g.NX * (jy + g.NY * jz));
+    const uint32_t vs = in ? 4u * ((fwd ? (uint32_t)n : j) * 128u + 9u * (uint32_t)blk) : GAL_OOB;
+    const uint32_t vx = in ? 12u * j : GAL_OOB;
+    float b[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) b[q] = gal_ld(rs, vs + 4u * q, 0u);
+    if (!fwd) {                                        // backward: the transpose
+      float u;
+      u = b[1]; b[1] = b[3]; b[3] = u;
+      u = b[2]; b[2] = b[6]; b[6] = u;
+      u = b[5]; b[5] = b[7]; b[7] = u;
+    }
+    float s0 = 1.0f, s1 = 1.0f, s2 = 1.0f;
+    if (MODE == EPI_CHEB0) {                           // x holds b; the input is c0 D^-1 b
+      s0 = c0f * gal_ld(rdi, vx, 0u);
+      s1 = c0f * gal_ld(rdi, vx + 4u, 0u);
+      s2 = c0f * gal_ld(rdi, vx + 8u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const float x0 = gal_ld(rx, vx, xoff[k]) * s0;
+      const float x1 = gal_ld(rx, vx + 4u, xoff[k]) * s1;
+      const float x2 = gal_ld(rx, vx + 8u, xoff[k]) * s2;
+      acc[k][0] += b[0] * x0 + b[1] * x1 + b[2] * x2;
+      acc[k][1] += b[3] * x0 + b[4] * x1 + b[5] * x2;
+      acc[k][2] += b[6] * x0 + b[7] * x1 + b[8] * x2;
     }
   }
   // sum over the node's 8 lanes (aligned groups of 8 within the wavefront)
@@ -300,21 +330,8 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
 // the GPU that way: the blocks are stored entry-major (S[e * n_nodes + n],
 // e = blk * 9 + 3 r + c), so each block entry and each neighbour's vector is
 // one coalesced load of consecutive nodes, and the sum over the 27 products
-// stays in the lane (no butterfly).  One loop trip takes the forward block
-// at n + d and the backward one at n - d (a rolled loop: unrolled, the 27
-// products' loads would not fit the register file).  Loads go through
-// buffer resources: the per-lane byte offset is the node's, the uniform
-// part (block entry, column) is a scalar offset, and a neighbour outside the
-// grid reads at an offset past the buffer, which returns 0.  Byte offsets
-// are 32-bit (the launcher checks the arrays are below 2^31 bytes).
-using GalRsrc = __amdgpu_buffer_rsrc_t;
-constexpr uint32_t GAL_OOB = 0x80000000u;
-__device__ __forceinline__ GalRsrc gal_rsrc(const void* p, uint32_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float gal_ld(GalRsrc r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
+// stays in the lane (no butterfly).  The block entry and the column are
+// uniform, so they go in the loads' scalar offsets.
 
 // Values of lane - 1 (wave_shr:1) and lane + 1 (wave_shl:1) of the
 // wavefront; lane 0, respectively lane 63, keeps `edge`.
@@ -521,10 +538,9 @@ static void gal_apply_nc(const Grid& g, const float* S32, bool by_entry, const u
 
 int gal_apply(const Grid& g, const float* S32, bool by_entry, const uint8_t* fixed, const float* x,
               float* y, int ncols, const Epilogue<float>& ep, hipStream_t st) {
-  // the one-node-per-lane kernel addresses its blocks and vectors with
-  // 32-bit byte offsets
-  VOX8_CHECK(!by_entry || ((int64_t)128 * 4 * g.n_nodes < ((int64_t)1 << 31) &&
-                           (int64_t)4 * ncols * g.n_dofs < ((int64_t)1 << 31)),
+  // both kernels address the blocks and vectors with 32-bit byte offsets
+  VOX8_CHECK((int64_t)128 * 4 * g.n_nodes < ((int64_t)1 << 31) &&
+                 (int64_t)4 * ncols * g.n_dofs < ((int64_t)1 << 31),
              "Galerkin level too large for 32-bit offsets");
   if (ncols <= 1) gal_apply_nc<1>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else if (ncols <= 2) gal_apply_nc<2>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
