@@ -286,16 +286,18 @@ __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __res
 // x = Ainv b for up to CN columns at once (blockIdx.y selects the column
 // group), so the inverse is read once per group (groups of 4 columns: more
 // groups give more workgroups, which this latency-bound product needs more
-// than fewer reads of the inverse).  Each wavefront owns 4 rows;
-// its lanes stride over the n entries (coalesced reads of the rows and of the
-// group's columns of b, which stay in cache), then reduce across the
-// wavefront.  16 rows per block, fixed summation order.
+// than fewer reads of the inverse).  A block computes 4 rows; its 4
+// wavefronts split the n entries (wavefront w takes k = 64 w + lane + 256 m,
+// coalesced reads of the rows and of the group's columns of b), reduce
+// across their lanes, and wavefront 0 adds the 4 partial sums in a fixed
+// order.
 template <int CN>
 __global__ void __launch_bounds__(256)
 k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float* __restrict__ b,
                   float* __restrict__ x, const int* __restrict__ skip) {
-  const int lane = threadIdx.x & 63;
-  const int row0 = blockIdx.x * 16 + (threadIdx.x >> 6) * 4;
+  __shared__ float part[4][4 * CN];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * 4;
   const int c0 = blockIdx.y * CN;
   const int nc = ncols - c0 < CN ? ncols - c0 : CN;
   float s[4][CN];
@@ -304,7 +306,7 @@ k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float*
 #pragma unroll
     for (int c = 0; c < CN; ++c) s[i][c] = 0.0f;
 #pragma unroll 4
-  for (int k = lane; k < n; k += 64) {
+  for (int k = 64 * w + lane; k < n; k += 256) {
     float bk[CN];
 #pragma unroll
     for (int c = 0; c < CN; ++c) bk[c] = c < nc ? b[(int64_t)(c0 + c) * n + k] : 0.0f;
@@ -322,10 +324,16 @@ k_mg_coarse_apply(int n, int ncols, const float* __restrict__ Ainv, const float*
       float v = s[i][c];
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      const int col = c0 + c;
-      if (lane == 0 && row0 + i < n && c < nc && !(skip && skip[col]))
-        x[(int64_t)col * n + row0 + i] = v;
+      if (lane == 0) part[w][i * CN + c] = v;
     }
+  __syncthreads();
+  if (threadIdx.x < 4 * CN) {
+    const int i = threadIdx.x / CN, c = threadIdx.x % CN;
+    const float v = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) +
+                    part[3][threadIdx.x];
+    const int col = c0 + c;
+    if (row0 + i < n && c < nc && !(skip && skip[col])) x[(int64_t)col * n + row0 + i] = v;
+  }
 }
 
 __global__ void k_f64_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b,
@@ -772,7 +780,7 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
-    const unsigned rb = (unsigned)((mg.n_coarse + 15) / 16);
+    const unsigned rb = (unsigned)((mg.n_coarse + 3) / 4);
     hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, (ncols + 3) / 4), dim3(256), 0, st,
                        mg.n_coarse, ncols, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
     VOX8_HIP(hipGetLastError());
