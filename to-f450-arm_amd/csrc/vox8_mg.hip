@@ -354,6 +354,18 @@ __global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __r
     b[off + i] = (double)a[off + i];
 }
 
+// xs += d per column: the last Chebyshev step of a post-smoothing sweep (its
+// residual and next direction are never read, so the operator is not
+// applied).  d is zero at fixed DOFs, as every smoothing step leaves it.
+__global__ void k_mg_add_cols(int64_t n, const float* __restrict__ d, float* __restrict__ xs,
+                              const int* __restrict__ skip) {
+  if (skip && skip[blockIdx.y]) return;
+  const int64_t off = (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    xs[off + i] += d[off + i];
+}
+
 // lambda_max(D^-1 K) by Lanczos in the D inner product <a, b>_D = sum a_i b_i / dinv_i,
 // in which D^-1 K is self-adjoint (fixed DOFs, dinv = 0, are outside the space).
 // Lanczos vectors are FP32, every sum FP64; alpha_j, beta_j stay on the device
@@ -749,6 +761,14 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
   const float* in = from_zero ? L.b : L.d;
   float* out = from_zero ? L.d : L.d2;
   for (int k = 0; k < nu; ++k) {
+    if (!from_zero && k == nu - 1) {
+      // the sweep's last step after a coarse correction: only xs += d is
+      // read afterwards (the cycle returns xs), so K d is not formed
+      hipLaunchKernelGGL(k_mg_add_cols, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st, L.g.n_dofs, in,
+                         L.x, skip);
+      VOX8_HIP(hipGetLastError());
+      break;
+    }
     const double rho1 = 1.0 / (2.0 * sigma - rho);
     Epilogue<float> ep;
     ep.mode = (k == 0 && from_zero) ? EPI_CHEB0 : EPI_CHEB;
