@@ -6,7 +6,8 @@ CPU (symmetry, positivity, contraction) and compared with the device kernel:
 levels coarsened 2:1 (ceil), factor = mean of 8 children, element matrix
 2^l Ke, trilinear P (masked by fixed DOFs on both sides; a coarse DOF
 is fixed when any fine node of its 3x3x3 interpolation stencil has it fixed), R = P^T, Chebyshev
-smoothing of D^-1 K on [lam_max/alpha, lam_max], dense coarsest solve; cycle
+smoothing of D^-1 K on [lam_max/alpha, lam_max] (nu steps, nu_coarse from
+level nu_coarse_level on), dense coarsest solve; cycle
 index gamma (1 = V, 2 = W with a fresh residual after each coarse correction).
 galerkin=True replaces the rediscretised coarse operators by P^T K P (sparse
 matrix products), the device's Galerkin option (vox8_set_mg_galerkin).
@@ -56,8 +57,9 @@ def prolongation(g_f, g_c, free_f, free_c):
 
 class VCycle:
     def __init__(self, dims, f, Ke, fixed_nodes_mask, active=None, nu=2, alpha=20.0, coarse_dofs=700,
-                 gamma=2, w_min_dofs=0, galerkin=False):
+                 gamma=2, w_min_dofs=0, galerkin=False, nu_coarse=0, nu_coarse_level=1):
         self.nu, self.alpha, self.gamma, self.w_min_dofs = nu, alpha, gamma, w_min_dofs
+        self.nu_coarse, self.nu_coarse_level = nu_coarse, nu_coarse_level
         nx, ny, nz = dims
         fixed = fixed_nodes_mask.astype(np.uint8) * 7 if fixed_nodes_mask.dtype == bool else fixed_nodes_mask
         act = np.ones(nx * ny * nz, bool) if active is None else active.astype(bool)
@@ -145,13 +147,14 @@ class VCycle:
         L = self.levels[l]
         if l == len(self.levels) - 1:
             return np.where(L["free"], L["inv"] @ b, 0.0)
+        nu = self.nu_coarse if (self.nu_coarse > 0 and l >= self.nu_coarse_level) else self.nu
         theta = 0.5 * (L["lam_max"] + L["lam_max"] / self.alpha)
-        x, r = self.smooth(L, np.zeros_like(b), b.copy(), L["dinv"] * b / theta, self.nu)
+        x, r = self.smooth(L, np.zeros_like(b), b.copy(), L["dinv"] * b / theta, nu)
         g = self.gamma if (l + 1 < len(self.levels) - 1 and L["K"].shape[0] >= self.w_min_dofs) else 1
         for _ in range(g):
             x = x + L["P"] @ self(L["P"].T @ r, l + 1)
             r = b - L["K"] @ x
-        x, _ = self.smooth(L, x, r, L["dinv"] * r / theta, self.nu)
+        x, _ = self.smooth(L, x, r, L["dinv"] * r / theta, nu)
         return x
 
 

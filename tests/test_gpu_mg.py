@@ -32,10 +32,16 @@ def _setup(shape, seed, hole):
 @pytest.mark.parametrize("shape,hole,gamma,wmin", [((24, 6, 5), 0.0, 2, 0), ((33, 9, 7), 0.1, 2, 0),
                                                    ((33, 9, 7), 0.1, 1, 0), ((70, 18, 14), 0.05, 2, 0),
                                                    ((70, 18, 14), 0.05, 2, 10000)])
-def test_vcycle_matches_reference(gpu, shape, hole, gamma, wmin):
+@pytest.mark.parametrize("nuc", [None, (0, 1), (3, 1)])
+def test_vcycle_matches_reference(gpu, shape, hole, gamma, wmin, nuc):
+    """nuc: (steps, from level) set through vox8_set_mg_coarse_smoothing;
+    None keeps the library default (4 steps from level 2)."""
     from mg_reference import VCycle
     geom, fem, rho, _lib = _setup(shape, 3, hole)
     fem._call("vox8_set_mg_options", 2, 20.0, 700, gamma, wmin)
+    if nuc is not None:
+        fem._call("vox8_set_mg_coarse_smoothing", *nuc)
+    nu_c, nu_l = nuc if nuc is not None else (4, 2)
     nl = C.c_int()
     dims = (C.c_int * 48)()
     lam = (C.c_double * 16)()
@@ -43,7 +49,8 @@ def test_vcycle_matches_reference(gpu, shape, hole, gamma, wmin):
     f = np.zeros(geom.n_elements)
     f[fem.active_elems] = fem.E_MIN_FRAC + (1 - fem.E_MIN_FRAC) * rho ** 3
     V = VCycle((geom.nx, geom.ny, geom.nz), f, fem.Ke_solid, fem.node_fixed,
-               active=fem.elem_active_grid, alpha=20.0, gamma=gamma, w_min_dofs=wmin)
+               active=fem.elem_active_grid, alpha=20.0, gamma=gamma, w_min_dofs=wmin,
+               nu_coarse=nu_c, nu_coarse_level=nu_l)
     assert [L["dims"] for L in V.levels] == [tuple(dims[3 * i:3 * i + 3]) for i in range(nl.value)]
     for l, L in enumerate(V.levels[:-1]):
         L["lam_max"] = lam[l]
@@ -90,7 +97,7 @@ def test_galerkin_cycle_matches_reference(gpu, monkeypatch, shape, hole, gamma, 
     f[fem.active_elems] = fem.E_MIN_FRAC + (1 - fem.E_MIN_FRAC) * rho ** 3
     V = VCycle((geom.nx, geom.ny, geom.nz), f, fem.Ke_solid, fem.node_fixed,
                active=fem.elem_active_grid, alpha=20.0, gamma=gamma, w_min_dofs=0,
-               coarse_dofs=coarse, galerkin=True)
+               coarse_dofs=coarse, galerkin=True, nu_coarse=4, nu_coarse_level=2)
     assert [L["dims"] for L in V.levels] == [tuple(dims[3 * i:3 * i + 3]) for i in range(nl.value)]
     for l, L in enumerate(V.levels[:-1]):
         assert abs(lam[l] / L["lam_max"] - 1) < 0.03     # Lanczos estimate vs exact (both x1.1)

@@ -302,6 +302,14 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   if (const char* e = getenv("VOX8_MG_ALPHA")) c->mg.alpha = std::max(1.5, atof(e));
   if (const char* e = getenv("VOX8_MG_MARGIN")) c->mg.lam_margin = std::max(1.0, atof(e));
   if (const char* e = getenv("VOX8_MG_VTOP")) c->mg.v_top = std::max(0, atoi(e));
+  if (const char* e = getenv("VOX8_MG_NU_LEVELS")) {          // "n0,n1,...": steps per level
+    int l = 0;
+    for (const char* q = e; *q && l < 8; ++l) {
+      c->mg.nu_level[l] = std::max(0, std::min(16, atoi(q)));
+      while (*q && *q != ',') ++q;
+      if (*q == ',') ++q;
+    }
+  }
   if (const char* e = getenv("VOX8_MG_LANE_NODES")) c->mg.lane_nodes = std::max(1LL, atoll(e));
   c->st = (hipStream_t)hip_stream;
   int s = 0;
@@ -446,6 +454,16 @@ int vox8_set_mg_options(vox8_ctx* c, int nu, double alpha, int coarse_dofs, int 
   c->mg.coarse_dofs = coarse_dofs;
   c->mg_version = ~0ull;
   ++c->mg.epoch;
+  return OK;
+}
+
+int vox8_set_mg_coarse_smoothing(vox8_ctx* c, int nu_coarse, int from_level) {
+  VOX8_CHECK(c != nullptr, "null context");
+  VOX8_CHECK(nu_coarse >= 0 && nu_coarse <= 16 && from_level >= 1,
+             "coarse smoothing out of range (nu_coarse 0..16, from_level >= 1)");
+  c->mg.nu_coarse = nu_coarse;
+  c->mg.nu_coarse_level = from_level;
+  ++c->mg.epoch;                       // captured cycles hold the old step counts
   return OK;
 }
 

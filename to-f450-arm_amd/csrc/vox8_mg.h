@@ -52,8 +52,10 @@ struct MultiGrid {
   std::vector<MGLevel> lv;
   int ncols = 0;               // capacity of the work vectors
   int nu = 2;                  // Chebyshev smoothing steps before and after
-  int nu_coarse = 0;           // the same on levels >= nu_coarse_level (0: nu)
-  int nu_coarse_level = 1;
+  int nu_coarse = 4;           // the same on levels >= nu_coarse_level (0: nu); the smaller
+  int nu_coarse_level = 2;     // levels' kernels are latency-bound, so more steps there cost
+                               // little against the coarse corrections they improve
+  int nu_level[8] = {};        // per-level override of the two rules above (0: the rules)
   int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
   int v_top = 0;               // the finest v_top levels take one correction (V) whatever gamma
   int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when

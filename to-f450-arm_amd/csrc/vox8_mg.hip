@@ -366,6 +366,18 @@ __global__ void k_mg_add_cols(int64_t n, const float* __restrict__ d, float* __r
     xs[off + i] += d[off + i];
 }
 
+// The same step on the finest level, where the cycle's result leaves as FP64:
+// z = (double)(xs + d), the FP32 sum widened (the bits the separate add and
+// conversion give), without writing xs back.
+__global__ void k_mg_add_cols_out64(int64_t n, const float* __restrict__ d, const float* __restrict__ xs,
+                                    double* __restrict__ z, const int* __restrict__ skip) {
+  if (skip && skip[blockIdx.y]) return;
+  const int64_t off = (int64_t)blockIdx.y * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    z[off + i] = (double)(xs[off + i] + d[off + i]);
+}
+
 // lambda_max(D^-1 K) by Lanczos in the D inner product <a, b>_D = sum a_i b_i / dinv_i,
 // in which D^-1 K is self-adjoint (fixed DOFs, dinv = 0, are outside the space).
 // Lanczos vectors are FP32, every sum FP64; alpha_j, beta_j stay on the device
@@ -754,7 +766,7 @@ struct ChebCoef { double theta, delta, sigma; };
 // direction itself; otherwise the direction left in L.d by a residual pass is
 // continued from the current x, r.
 static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st, bool from_zero,
-                  const int* skip) {
+                  const int* skip, double* out64 = nullptr) {
   const double lmax = L.lam_max, lmin = lmax / alpha;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
   double rho = 1.0 / sigma;
@@ -764,8 +776,12 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
     if (!from_zero && k == nu - 1) {
       // the sweep's last step after a coarse correction: only xs += d is
       // read afterwards (the cycle returns xs), so K d is not formed
-      hipLaunchKernelGGL(k_mg_add_cols, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st, L.g.n_dofs, in,
-                         L.x, skip);
+      if (out64)
+        hipLaunchKernelGGL(k_mg_add_cols_out64, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st,
+                           L.g.n_dofs, in, (const float*)L.x, out64, skip);
+      else
+        hipLaunchKernelGGL(k_mg_add_cols, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st, L.g.n_dofs,
+                           in, L.x, skip);
       VOX8_HIP(hipGetLastError());
       break;
     }
@@ -796,7 +812,10 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
 // 8 nu smoothing steps from zero when it is too large for one.  With the same
 // polynomial before and after and symmetric transfers the operator is
 // symmetric, as PCG requires.
-static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* skip) {
+// out64 (finest level only): where the cycle's result goes as FP64; the
+// post-smoothing sweep's last step writes it (L.x is then not updated).
+static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* skip,
+                    double* out64 = nullptr) {
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
@@ -808,7 +827,8 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   }
   const double theta = 0.5 * (L.lam_max + L.lam_max / mg.alpha);
   if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true, skip);   // no coarse grid below
-  const int nu = (l >= mg.nu_coarse_level && mg.nu_coarse > 0) ? mg.nu_coarse : mg.nu;
+  const int nu = (l < 8 && mg.nu_level[l] > 0) ? mg.nu_level[l]
+                 : (l >= mg.nu_coarse_level && mg.nu_coarse > 0) ? mg.nu_coarse : mg.nu;
   VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st, true, skip));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
@@ -832,7 +852,7 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     ep.skip = skip;
     VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
   }
-  return smooth(L, nu, mg.alpha, ncols, st, false, skip);
+  return smooth(L, nu, mg.alpha, ncols, st, false, skip, out64);
 }
 
 static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,
@@ -841,10 +861,15 @@ static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncol
   hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b,
                      skip);
   VOX8_HIP(hipGetLastError());
-  VOX8_TRY(mg_cycle(mg, 0, ncols, st, skip));
-  hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
-                     (const float*)F.x, z, skip);
-  VOX8_HIP(hipGetLastError());
+  // With a coarser level below, the finest level's cycle ends with a
+  // post-smoothing sweep, whose last step writes z directly.
+  const bool fused_out = mg.lv.size() > 1;
+  VOX8_TRY(mg_cycle(mg, 0, ncols, st, skip, fused_out ? z : nullptr));
+  if (!fused_out) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
+                       (const float*)F.x, z, skip);
+    VOX8_HIP(hipGetLastError());
+  }
   return OK;
 }
 
