@@ -373,28 +373,30 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
 // Implicit updates (K P' = [KW KP] C[K:]) carry the rounding of every earlier
 // step, amplified by the cancellation in P' near convergence; the error grows
 // geometrically and the Rayleigh-Ritz then picks diverging combinations.
+// One row per thread (no grid-stride loop): with a loop the compiler keeps
+// all of C in registers across iterations (M*K doubles: one wave per SIMD,
+// and scratch spills from k = 8 on); here C is read from LDS as it is used.
 template <int K, int NB>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB * K * K > 150 ? 2 : 4)))
 k_ritz_combine(int64_t n, double* __restrict__ S, const double* __restrict__ Cg) {
   constexpr int M = NB * K;
   __shared__ double C[M * K];
   for (int e = threadIdx.x; e < M * K; e += blockDim.x) C[e] = Cg[e];
   __syncthreads();
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    double v[M];
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double v[M];
 #pragma unroll
-    for (int i = 0; i < M; ++i) v[i] = S[(int64_t)i * n + r];
+  for (int i = 0; i < M; ++i) v[i] = S[(int64_t)i * n + r];
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      double x = 0.0, p = 0.0;
+  for (int j = 0; j < K; ++j) {
+    double x = 0.0, p = 0.0;
 #pragma unroll
-      for (int i = 0; i < K; ++i) x += v[i] * C[i * K + j];
+    for (int i = 0; i < K; ++i) x += v[i] * C[i * K + j];
 #pragma unroll
-      for (int i = K; i < M; ++i) p += v[i] * C[i * K + j];
-      S[(int64_t)j * n + r] = x + p;
-      if (NB > 1) S[(int64_t)(2 * K + j) * n + r] = p;
-    }
+    for (int i = K; i < M; ++i) p += v[i] * C[i * K + j];
+    S[(int64_t)j * n + r] = x + p;
+    if (NB > 1) S[(int64_t)(2 * K + j) * n + r] = p;
   }
 }
 
@@ -479,7 +481,7 @@ static int gram_and_rr(LobpcgWork& w, int m, int k, hipStream_t st, double drop_
 
 template <int K>
 static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStream_t st) {
-  const dim3 g(grid_blocks(w.n)), b(256);
+  const dim3 g((unsigned)((w.n + 255) / 256)), b(256);
   if (nb == 1) hipLaunchKernelGGL((k_ritz_combine<K, 1>), g, b, 0, st, w.n, w.S, C);
   else if (nb == 2) hipLaunchKernelGGL((k_ritz_combine<K, 2>), g, b, 0, st, w.n, w.S, C);
   else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, C);

@@ -128,15 +128,19 @@ __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
 
 // fine x += P coarse x (trilinear), fixed fine DOFs untouched.  Branch-free:
 // on an axis where the fine node coincides with a coarse one the second coarse
-// index equals the first and its term is not added.
+// index equals the first and its term is not added.  ADD_D: the coarse
+// level's post-smoothing left its last step xc += dc to this kernel, which
+// reads xc + dc (the same FP32 sum the separate add would have stored).
+template <bool ADD_D>
 __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
-                                 const uint8_t* __restrict__ fixf, float* __restrict__ xf,
-                                 const int* __restrict__ skip) {
+                                 const float* __restrict__ dc, const uint8_t* __restrict__ fixf,
+                                 float* __restrict__ xf, const int* __restrict__ skip) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= gf.n_nodes) return;
   const int col = blockIdx.y;
   if (skip && skip[col]) return;
   xc += (int64_t)col * gc.n_dofs;
+  if (ADD_D) dc += (int64_t)col * gc.n_dofs;
   xf += (int64_t)col * gf.n_dofs;
   int i, j, k;
   node_ijk(gf, n, i, j, k);
@@ -151,7 +155,11 @@ __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
       for (int cx = 0; cx < 2; ++cx) {
         const bool use = (cz == 0 || z0 != z1) && (cy == 0 || y0 != y1) && (cx == 0 || x0 != x1);
         const int64_t c = (cx ? x1 : x0) + (int64_t)gc.NX * ((cy ? y1 : y0) + (int64_t)gc.NY * (cz ? z1 : z0));
-        const F3 v = ld_f3(xc + 3 * c);
+        F3 v = ld_f3(xc + 3 * c);
+        if (ADD_D) {
+          const F3 d = ld_f3(dc + 3 * c);
+          v.x += d.x; v.y += d.y; v.z += d.z;
+        }
         s0 += use ? v.x : 0.0f;
         s1 += use ? v.y : 0.0f;
         s2 += use ? v.z : 0.0f;
@@ -256,25 +264,45 @@ __global__ void k_gjb_w(int n, int k0, int b, const double* __restrict__ A, cons
   W[e] = s;
 }
 
-__global__ void k_gjb_update(int n, int k0, int b, const double* __restrict__ A, double* __restrict__ B,
-                             const double* __restrict__ D, const double* __restrict__ W) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)n * n) return;
-  const int i = (int)(e / n), j = (int)(e % n);
-  const bool iK = i >= k0 && i < k0 + b, jK = j >= k0 && j < k0 + b;
-  double v;
-  if (iK && jK) {
-    v = D[(i - k0) * b + (j - k0)];
-  } else if (iK) {
-    v = 0.0;
-    for (int m = 0; m < b; ++m) v += D[(i - k0) * b + m] * A[(int64_t)(k0 + m) * n + j];
-  } else if (jK) {
-    v = -W[(int64_t)i * b + (j - k0)];
-  } else {
-    v = A[e];
-    for (int m = 0; m < b; ++m) v -= W[(int64_t)i * b + m] * A[(int64_t)(k0 + m) * n + j];
+// Every entry of the next iterate.  A thread owns column j of a group of
+// GJ_ROWS rows (blockIdx.y): the panel rows A_K[:, j] are loaded once into
+// registers and reused for every row of the group, while the row's W (or D)
+// entries are the same for the whole wavefront (uniform loads); the sums run
+// over m in the same order as a per-entry loop, so the values are unchanged.
+constexpr int GJ_ROWS = 8;
+__global__ void __launch_bounds__(256)
+k_gjb_update(int n, int k0, int b, const double* __restrict__ A, double* __restrict__ B,
+             const double* __restrict__ D, const double* __restrict__ W) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int i0 = blockIdx.y * GJ_ROWS;
+  const bool jK = j >= k0 && j < k0 + b;
+  double akj[GJ_B];
+  if (!jK) {
+#pragma unroll
+    for (int m = 0; m < GJ_B; ++m) akj[m] = m < b ? A[(int64_t)(k0 + m) * n + j] : 0.0;
   }
-  B[e] = v;
+#pragma unroll
+  for (int r = 0; r < GJ_ROWS; ++r) {
+    const int i = i0 + r;
+    if (i >= n) break;
+    const bool iK = i >= k0 && i < k0 + b;
+    double v;
+    if (jK) {
+      v = iK ? D[(i - k0) * b + (j - k0)] : -W[(int64_t)i * b + (j - k0)];
+    } else if (iK) {
+      v = 0.0;
+#pragma unroll
+      for (int m = 0; m < GJ_B; ++m)                    // b is uniform: a uniform guard
+        if (m < b) v += D[(i - k0) * b + m] * akj[m];
+    } else {
+      v = A[(int64_t)i * n + j];
+#pragma unroll
+      for (int m = 0; m < GJ_B; ++m)
+        if (m < b) v -= W[(int64_t)i * b + m] * akj[m];
+    }
+    B[(int64_t)i * n + j] = v;
+  }
 }
 
 __global__ void k_to_f32_n(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -727,7 +755,7 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
                          (const double*)(mg.lv.size() == 1 ? fK0 : C.fK64), (const uint8_t*)C.fixed, dKe,
                          C.scale, mg.A);
     double* diag0 = mg.red;
-    const unsigned gs = (unsigned)((n + 255) / 256), ge = (unsigned)(((int64_t)n * n + 255) / 256);
+    const unsigned gs = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_gj_diag, dim3(gs), dim3(256), 0, st, n, (const double*)mg.A, diag0);
     double* a = mg.A;
     double* b = mg.A2;
@@ -739,8 +767,8 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
                          (const double*)diag0, mg.scal + 7);
       hipLaunchKernelGGL(k_gjb_w, dim3((unsigned)(((int64_t)n * bs + 255) / 256)), dim3(256), 0, st, n,
                          k0, bs, (const double*)a, (const double*)D, W);
-      hipLaunchKernelGGL(k_gjb_update, dim3(ge), dim3(256), 0, st, n, k0, bs, (const double*)a, b,
-                         (const double*)D, (const double*)W);
+      hipLaunchKernelGGL(k_gjb_update, dim3(gs, (unsigned)((n + GJ_ROWS - 1) / GJ_ROWS)), dim3(256), 0,
+                         st, n, k0, bs, (const double*)a, b, (const double*)D, (const double*)W);
       std::swap(a, b);
     }
     hipLaunchKernelGGL(k_to_f32_n, dim3(nb((int64_t)n * n)), dim3(256), 0, st, (int64_t)n * n,
@@ -765,8 +793,11 @@ struct ChebCoef { double theta, delta, sigma; };
 // at x = 0, r = b and the first step (EPI_CHEB0) reads b and forms the first
 // direction itself; otherwise the direction left in L.d by a residual pass is
 // continued from the current x, r.
+// pending (coarse levels): the post-smoothing sweep's last step xs += d is
+// not launched; *pending receives d, and the prolongation that reads xs adds
+// it on the fly.
 static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st, bool from_zero,
-                  const int* skip, double* out64 = nullptr) {
+                  const int* skip, double* out64 = nullptr, const float** pending = nullptr) {
   const double lmax = L.lam_max, lmin = lmax / alpha;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
   double rho = 1.0 / sigma;
@@ -776,6 +807,10 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
     if (!from_zero && k == nu - 1) {
       // the sweep's last step after a coarse correction: only xs += d is
       // read afterwards (the cycle returns xs), so K d is not formed
+      if (pending) {
+        *pending = in;
+        break;
+      }
       if (out64)
         hipLaunchKernelGGL(k_mg_add_cols_out64, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st,
                            L.g.n_dofs, in, (const float*)L.x, out64, skip);
@@ -814,8 +849,10 @@ static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t
 // symmetric, as PCG requires.
 // out64 (finest level only): where the cycle's result goes as FP64; the
 // post-smoothing sweep's last step writes it (L.x is then not updated).
+// pending (coarser levels): as in smooth(); stays null when the level ends
+// without such a step (dense coarsest level).
 static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* skip,
-                    double* out64 = nullptr) {
+                    double* out64 = nullptr, const float** pending = nullptr) {
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
@@ -837,9 +874,15 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
                        L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b, skip);
     VOX8_HIP(hipGetLastError());
-    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip));
-    hipLaunchKernelGGL(k_mg_prolong_add, dim3((unsigned)((L.g.n_nodes + 255) / 256), ncols), dim3(256), 0,
-                       st, Cn.g, L.g, (const float*)Cn.x, (const uint8_t*)L.fixed, L.x, skip);
+    const float* dpend = nullptr;
+    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip, nullptr, &dpend));
+    const dim3 pgrid((unsigned)((L.g.n_nodes + 255) / 256), ncols);
+    if (dpend)
+      hipLaunchKernelGGL(k_mg_prolong_add<true>, pgrid, dim3(256), 0, st, Cn.g, L.g, (const float*)Cn.x,
+                         dpend, (const uint8_t*)L.fixed, L.x, skip);
+    else
+      hipLaunchKernelGGL(k_mg_prolong_add<false>, pgrid, dim3(256), 0, st, Cn.g, L.g, (const float*)Cn.x,
+                         (const float*)nullptr, (const uint8_t*)L.fixed, L.x, skip);
     VOX8_HIP(hipGetLastError());
     Epilogue<float> ep;
     ep.mode = EPI_RESID;
@@ -852,7 +895,7 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     ep.skip = skip;
     VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
   }
-  return smooth(L, nu, mg.alpha, ncols, st, false, skip, out64);
+  return smooth(L, nu, mg.alpha, ncols, st, false, skip, out64, pending);
 }
 
 static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,

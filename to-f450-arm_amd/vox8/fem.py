@@ -213,8 +213,13 @@ class FEM3D:
     def __del__(self):
         ctx = getattr(self, "_ctx", None)
         if ctx is not None and ctx.value:
-            _lib.release(ctx)
             self._ctx = None
+            try:
+                _lib.release(ctx)
+            except (AttributeError, TypeError):
+                # interpreter shutdown: module globals are already cleared
+                # and the process is ending, so the context goes with it
+                pass
 
     def _call(self, name, *args):
         if self._ctx is None:
