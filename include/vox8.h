@@ -68,7 +68,10 @@ int vox8_set_tip_mass(vox8_ctx* ctx, const int64_t* h_nodes, int n_nodes, double
 
 /* Element densities d_rho[n_elem] (element grid order).  Computes the SIMP
  * factor e_min + (1-e_min) rho^p and the Pedersen mass factor on the device
- * (src/fem.py:316-320 assemble_K, 334-349 _mass_simp_factor). */
+ * (src/fem.py:316-320 assemble_K, 334-349 _mass_simp_factor).  Returns when
+ * the factors are formed; factors equal to the current ones (the same
+ * density set again) leave what is derived from them (multigrid hierarchy,
+ * Jacobi diagonal) in place. */
 int vox8_set_density(vox8_ctx* ctx, const double* d_rho, double p, double e_min,
                      double mass_threshold, double mass_q);
 
@@ -146,6 +149,20 @@ int vox8_set_mg_options(vox8_ctx* ctx, int nu, double alpha, int coarse_dofs, in
  * better coarse corrections cut PCG and LOBPCG iterations by 10-15 % on the
  * 440x64x48 SIMP iterations.  No reference counterpart. */
 int vox8_set_mg_coarse_smoothing(vox8_ctx* ctx, int nu_coarse, int from_level);
+/* ctx's multigrid preconditioner takes the per-density operators of src's
+ * hierarchy (Galerkin blocks or level factors, Jacobi diagonals, smoothing
+ * bounds, coarsest inverse) by a device copy on ctx's stream instead of
+ * computing its own: for a second context of the same model solving with the
+ * same density at the same time (a concurrent modal solve).  The caller sets
+ * the same densities on both and brings src up to date first
+ * (vox8_prepare_multigrid); src must not be updated while ctx's solve runs,
+ * and must outlive the sharing (src NULL ends it).  ctx keeps its own work
+ * vectors and captured cycles.  No reference counterpart. */
+int vox8_share_multigrid(vox8_ctx* ctx, vox8_ctx* src);
+/* Set up (or update for the current density) the multigrid hierarchy with
+ * work space for ncols columns; returns when the device work is done.  No
+ * reference counterpart. */
+int vox8_prepare_multigrid(vox8_ctx* ctx, int ncols);
 /* Coarse levels of the multigrid preconditioner: 1 = Galerkin
  * A_{l+1} = P^T A_l P (per-element 24x24 matrices, applied in node-block
  * form; the default), 0 = rediscretised (the box element of each level with

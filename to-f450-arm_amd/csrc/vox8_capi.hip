@@ -54,6 +54,8 @@ struct vox8_ctx {
   unsigned int* red_counters = nullptr;  // 64
   double* red_out = nullptr;        // 64 doubles
   double* host = nullptr;           // pinned 64 doubles
+  unsigned int* d_changed = nullptr;  // vox8_set_density: did any element factor change
+  unsigned int* h_changed = nullptr;  // pinned mirror
   // PCG
   PcgWork<double> w64;
   PcgWork<float> w32;
@@ -70,6 +72,7 @@ struct vox8_ctx {
   MultiGrid mg;
   std::vector<uint8_t> h_active, h_fixed;
   uint64_t density_version = 0, mg_version = ~0ull;
+  vox8_ctx* mg_src = nullptr;       // vox8_share_multigrid: operators copied from this context
   // OC / sensitivities scratch
   double* d_oc = nullptr;
   double* d_tail = nullptr;         // TAIL_STATE doubles
@@ -245,7 +248,15 @@ int ensure_mg(vox8_ctx* c, int ncols) {
   }
   VOX8_TRY(mg_reserve(c->mg, ncols, al));
   if (c->mg_version != c->density_version) {
-    VOX8_TRY(mg_update(c->mg, c->fK, c->dKe, c->st));
+    if (c->mg_src) {
+      const vox8_ctx* s = c->mg_src;
+      VOX8_CHECK(s->mg.built && s->mg.updated && s->mg_version == s->density_version,
+                 "shared multigrid: the source context's hierarchy is not set up for its current "
+                 "density (vox8_prepare_multigrid on it first)");
+      VOX8_TRY(mg_copy_operators(c->mg, s->mg, c->st));
+    } else {
+      VOX8_TRY(mg_update(c->mg, c->fK, c->dKe, c->st));
+    }
     c->mg_version = c->density_version;
   }
   return OK;
@@ -302,6 +313,14 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
   if (const char* e = getenv("VOX8_MG_ALPHA")) c->mg.alpha = std::max(1.5, atof(e));
   if (const char* e = getenv("VOX8_MG_MARGIN")) c->mg.lam_margin = std::max(1.0, atof(e));
   if (const char* e = getenv("VOX8_MG_VTOP")) c->mg.v_top = std::max(0, atoi(e));
+  if (const char* e = getenv("VOX8_MG_GAMMA_LEVELS")) {       // "g0,g1,...": corrections per level
+    int l = 0;
+    for (const char* q = e; *q && l < 8; ++l) {
+      c->mg.gamma_level[l] = std::max(0, std::min(2, atoi(q)));
+      while (*q && *q != ',') ++q;
+      if (*q == ',') ++q;
+    }
+  }
   if (const char* e = getenv("VOX8_MG_NU_LEVELS")) {          // "n0,n1,...": steps per level
     int l = 0;
     for (const char* q = e; *q && l < 8; ++l) {
@@ -328,7 +347,8 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
       (s = dalloc(c, &c->d_kphi, ne)) ||
       (s = dalloc(c, &c->d_mphi, ne)) || (s = dalloc(c, &c->d_ce, ne)) ||
       (s = dalloc(c, &c->d_sums, 32)) || (s = dalloc(c, &c->d_wts, 32)) ||
-      (s = dalloc(c, &c->d_den, ne)) || (s = dalloc(c, &c->d_fmask, ne)))
+      (s = dalloc(c, &c->d_den, ne)) || (s = dalloc(c, &c->d_fmask, ne)) ||
+      (s = dalloc(c, &c->d_changed, 1)) || (s = halloc(c, &c->h_changed, 1)))
     return fail(s);
   // stream-ordered (a legacy-stream memset conflicts with graph captures in
   // progress on other threads)
@@ -467,6 +487,33 @@ int vox8_set_mg_coarse_smoothing(vox8_ctx* c, int nu_coarse, int from_level) {
   return OK;
 }
 
+int vox8_share_multigrid(vox8_ctx* c, vox8_ctx* src) {
+  VOX8_CHECK(c != nullptr && c != src, "share_multigrid: null context or a context sharing with itself");
+  VOX8_CHECK(src == nullptr || src->mg_src == nullptr, "share_multigrid: the source shares itself");
+  if (src) {
+    VOX8_CHECK(src->g.nx == c->g.nx && src->g.ny == c->g.ny && src->g.nz == c->g.nz &&
+                   src->device == c->device,
+               "share_multigrid: contexts of different grids or devices");
+  }
+  c->mg_src = src;
+  if (c->mg.operators_only != (src != nullptr)) {
+    c->mg.operators_only = src != nullptr;   // a sharing hierarchy holds no element matrices
+    c->mg.built = false;
+  }
+  c->mg_version = ~0ull;               // the next solve takes the operators (or updates itself)
+  return OK;
+}
+
+int vox8_prepare_multigrid(vox8_ctx* c, int ncols) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(ncols >= 1 && ncols <= 16, "prepare_multigrid: 1..16 columns");
+  VOX8_TRY(ensure_mg(c, ncols));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  return OK;
+  API_END
+}
+
 int vox8_set_mg_galerkin(vox8_ctx* c, int enable) {
   VOX8_CHECK(c != nullptr, "null context");
   if (c->mg.galerkin != (enable != 0)) {
@@ -529,11 +576,21 @@ int vox8_set_density(vox8_ctx* c, const double* d_rho, double p, double e_min, d
   API_BEGIN
   VOX8_TRY(require(c, false));
   VOX8_CHECK(d_rho != nullptr, "null density");
+  // The factors are compared with the ones they replace: setting the same
+  // density again (a caller that re-sends its density with every solve)
+  // keeps the density version, so nothing derived from it (the multigrid
+  // hierarchy, the Jacobi diagonal) is set up again.
+  VOX8_HIP(hipMemsetAsync(c->d_changed, 0, sizeof(unsigned int), c->st));
   VOX8_TRY(launch_element_factors(c->g, d_rho, c->d_active, p, e_min, mthr, mq, c->fK, c->fM,
-                                  c->fK32, c->fM32, c->st));
+                                  c->fK32, c->fM32, c->st, c->d_changed));
+  VOX8_HIP(hipMemcpyAsync(c->h_changed, c->d_changed, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                          c->st));
+  VOX8_HIP(hipStreamSynchronize(c->st));
+  if (!c->have_density || *c->h_changed) {
+    c->dinv_valid = false;
+    ++c->density_version;
+  }
   c->have_density = true;
-  c->dinv_valid = false;
-  ++c->density_version;
   return OK;
   API_END
 }

@@ -16,7 +16,8 @@ struct DotFuse {
 // ---- stencil (vox8_stencil_dense.hip) ----
 int launch_element_factors(const Grid& g, const double* rho, const uint8_t* active, double p,
                            double emin, double mthr, double mq, double* fK, double* fM,
-                           float* fK32, float* fM32, hipStream_t st);
+                           float* fK32, float* fM32, hipStream_t st,
+                           unsigned int* changed = nullptr);
 template <typename T>
 int launch_apply_dense(const Grid& g, int op, const T* fK, const T* fM, const T* Ke,
                        const T* Me, const uint8_t* fixed, const T* x, int64_t ldx, T* yK,

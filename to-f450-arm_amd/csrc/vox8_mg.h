@@ -56,6 +56,7 @@ struct MultiGrid {
   int nu_coarse_level = 2;     // levels' kernels are latency-bound, so more steps there cost
                                // little against the coarse corrections they improve
   int nu_level[8] = {};        // per-level override of the two rules above (0: the rules)
+  int gamma_level[8] = {};     // per-level override of the coarse-correction count (0: rules)
   int gamma = 2;               // coarse corrections per level (1 = V-cycle, 2 = W-cycle)
   int v_top = 0;               // the finest v_top levels take one correction (V) whatever gamma
   int64_t w_min_dofs = 5000;   // levels smaller than this take one correction (V) even when
@@ -65,6 +66,8 @@ struct MultiGrid {
                                // per lane; smaller ones spread a node over 8 lanes
   double alpha = 20.0;         // smoothing interval [lam_max/alpha, lam_max]
   bool galerkin = true;        // coarse levels P^T A P (element matrices); false: rediscretised
+  bool operators_only = false; // the per-density operators come from another hierarchy
+                               // (mg_copy_operators): no element matrices are allocated
   double lam_margin = 1.1;     // lambda_max = margin x the largest Lanczos Ritz value
   // coarsest level: dense inverse when small
   int n_coarse = 0;
@@ -78,6 +81,7 @@ struct MultiGrid {
   float* tmp32 = nullptr;      // fine-level input/output staging (ncols * n0)
   bool built = false;
   bool updated = false;
+  uint64_t updates = 0;        // completed mg_update / mg_copy_operators calls
   // captured cycles; epoch changes whenever a captured launch sequence would
   // (hierarchy, work vectors, coefficients, options)
   bool use_graphs = true;
@@ -123,6 +127,11 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
 int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t st);
 // Ensure work vectors for ncols columns.
 int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al);
+// Take the per-density operators of src (a hierarchy of the same grid, masks
+// and options, updated for the density dst is to use): Galerkin node blocks
+// or level factors, Jacobi diagonals, lambda_max and the coarsest inverse are
+// copied on st, so dst needs no update of its own.
+int mg_copy_operators(MultiGrid& dst, const MultiGrid& src, hipStream_t st);
 // z = cycle(r) (V or W, see MultiGrid::gamma) for ncols FP64 column-major
 // vectors (ld = fine n_dofs).  skip (device, per column, may be null):
 // non-zero columns are left untouched (z keeps its contents).

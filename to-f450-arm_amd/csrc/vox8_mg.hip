@@ -575,7 +575,10 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
       VOX8_HIP(hipMemcpyAsync(L.fixed, fix.data(), g.n_nodes, hipMemcpyHostToDevice, st));
       VOX8_HIP(hipStreamSynchronize(st));   // act / fix change on the next level
       if (mg.galerkin) {
-        VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 576 * g.n_elem)); L.E64 = (double*)p;
+        if (!mg.operators_only) {
+          VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * 576 * g.n_elem));
+          L.E64 = (double*)p;
+        }
         VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * 128 * g.n_nodes)); L.S32 = (float*)p;
         L.s_by_entry = g.n_nodes >= mg.lane_nodes;
       }
@@ -631,8 +634,10 @@ int mg_build(MultiGrid& mg, const Grid& g0, const uint8_t* h_active, const uint8
   void* p = nullptr;
   if (mg.n_coarse) {
     const size_t nn = (size_t)mg.n_coarse * mg.n_coarse;
-    VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A = (double*)p;
-    VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A2 = (double*)p;
+    if (!mg.operators_only) {
+      VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A = (double*)p;
+      VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * nn)); mg.A2 = (double*)p;
+    }
     VOX8_TRY(al.dev(al.ctx, &p, sizeof(float) * nn)); mg.Ainv = (float*)p;
   }
   VOX8_TRY(al.dev(al.ctx, &p, sizeof(double) * std::max<size_t>(6144, (size_t)mg.n_coarse * (GJ_B + 1)
@@ -662,6 +667,43 @@ int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al) {
   }
   mg.ncols = ncols;
   ++mg.epoch;
+  return OK;
+}
+
+int mg_copy_operators(MultiGrid& dst, const MultiGrid& src, hipStream_t st) {
+  VOX8_CHECK(dst.built && src.built && src.updated, "shared multigrid: hierarchies not set up");
+  bool same = dst.lv.size() == src.lv.size() && dst.n_coarse == src.n_coarse &&
+              dst.galerkin == src.galerkin;
+  for (size_t l = 0; same && l < dst.lv.size(); ++l) {
+    const Grid &a = dst.lv[l].g, &b = src.lv[l].g;
+    same = a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && (dst.lv[l].S32 != nullptr) == (src.lv[l].S32 != nullptr);
+  }
+  VOX8_CHECK(same, "shared multigrid: the two contexts' hierarchies differ (grid, options or "
+                   "coarse-level kind)");
+  for (size_t l = 0; l < dst.lv.size(); ++l) {
+    MGLevel& D = dst.lv[l];
+    const MGLevel& S = src.lv[l];
+    if (l > 0) {
+      if (S.S32) {
+        VOX8_HIP(hipMemcpyAsync(D.S32, S.S32, sizeof(float) * 128 * (size_t)D.g.n_nodes,
+                                hipMemcpyDeviceToDevice, st));
+      } else {
+        VOX8_HIP(hipMemcpyAsync(D.fK64, S.fK64, sizeof(double) * (size_t)D.g.n_elem,
+                                hipMemcpyDeviceToDevice, st));
+        VOX8_HIP(hipMemcpyAsync(D.fK, S.fK, sizeof(float) * (size_t)D.g.n_elem, hipMemcpyDeviceToDevice,
+                                st));
+      }
+    }
+    VOX8_HIP(hipMemcpyAsync(D.dinv, S.dinv, sizeof(float) * (size_t)D.g.n_dofs, hipMemcpyDeviceToDevice,
+                            st));
+    D.lam_max = S.lam_max;
+  }
+  if (dst.n_coarse)
+    VOX8_HIP(hipMemcpyAsync(dst.Ainv, src.Ainv, sizeof(float) * (size_t)dst.n_coarse * dst.n_coarse,
+                            hipMemcpyDeviceToDevice, st));
+  dst.updated = true;
+  ++dst.updates;
+  ++dst.epoch;
   return OK;
 }
 
@@ -783,6 +825,7 @@ int mg_update(MultiGrid& mg, const double* fK0, const double* dKe, hipStream_t s
     }
   }
   mg.updated = true;
+  ++mg.updates;
   ++mg.epoch;
   return OK;
 }
@@ -869,7 +912,8 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
   VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st, true, skip));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
-  const int g = (exact_below || L.g.n_dofs < mg.w_min_dofs || l < mg.v_top) ? 1 : mg.gamma;
+  int g = (exact_below || L.g.n_dofs < mg.w_min_dofs || l < mg.v_top) ? 1 : mg.gamma;
+  if (l < 8 && mg.gamma_level[l] > 0) g = mg.gamma_level[l];
   for (int j = 0; j < g; ++j) {
     hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
                        L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b, skip);

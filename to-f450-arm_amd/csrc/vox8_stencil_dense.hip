@@ -13,11 +13,14 @@ namespace vox8 {
 
 // f_K = e_min + (1-e_min) rho^p ; f_M = Pedersen factor (src/fem.py:320, 334-349).
 // Inactive elements get 0 for both, so they drop out of every product.
+// changed (may be null): set to 1 when any factor differs from the value it
+// replaces (bitwise), so a context can tell a repeated upload of the same
+// density from a new one.
 __global__ void k_element_factors(int64_t n_elem, const double* __restrict__ rho,
                                   const uint8_t* __restrict__ active, double p, double emin,
                                   double mthr, double mq, double* __restrict__ fK,
                                   double* __restrict__ fM, float* __restrict__ fK32,
-                                  float* __restrict__ fM32) {
+                                  float* __restrict__ fM32, unsigned int* __restrict__ changed) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_elem) return;
   double r = rho[e];
@@ -26,6 +29,9 @@ __global__ void k_element_factors(int64_t n_elem, const double* __restrict__ rho
     k = emin + (1.0 - emin) * pow(r, p);
     m = (r >= mthr) ? r : mthr * pow(fmax(r, 0.0) / mthr, mq);
   }
+  if (changed && (__double_as_longlong(fK[e]) != __double_as_longlong(k) ||
+                  __double_as_longlong(fM[e]) != __double_as_longlong(m)))
+    *changed = 1u;                          // every writer stores the same value
   fK[e] = k;
   fM[e] = m;
   fK32[e] = (float)k;
@@ -157,10 +163,10 @@ __global__ void k_jacobi(Grid g, const double* __restrict__ f, const double* __r
 // ---------------------------------------------------------------------------
 int launch_element_factors(const Grid& g, const double* rho, const uint8_t* active, double p,
                            double emin, double mthr, double mq, double* fK, double* fM,
-                           float* fK32, float* fM32, hipStream_t st) {
+                           float* fK32, float* fM32, hipStream_t st, unsigned int* changed) {
   const int B = 256;
   hipLaunchKernelGGL(k_element_factors, dim3((unsigned)((g.n_elem + B - 1) / B)), dim3(B), 0, st,
-                     g.n_elem, rho, active, p, emin, mthr, mq, fK, fM, fK32, fM32);
+                     g.n_elem, rho, active, p, emin, mthr, mq, fK, fM, fK32, fM32, changed);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

@@ -59,6 +59,8 @@ SIGNATURES = {
     "vox8_set_mg_graphs": [_vp, _i32],
     "vox8_set_mg_galerkin": [_vp, _i32],
     "vox8_set_mg_coarse_smoothing": [_vp, _i32, _i32],
+    "vox8_share_multigrid": [_vp, _vp],
+    "vox8_prepare_multigrid": [_vp, _i32],
     "vox8_stress": [_vp, _vp, _vp, _pf64, _pf64, _vp],
     "vox8_stress_invariants": [_vp, C.c_int64, _vp, _vp, _vp],
     "vox8_fatigue": [_vp, C.c_int64, _vp, _vp, _f64, _pf64, _pf64, _f64, _f64, _f64,

@@ -19,6 +19,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 import warnings
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -197,6 +198,8 @@ class FEM3D:
         self._density_key = None
         self._tip_key = None
         self.last_info = {}
+        self._mg_src = None                    # share_multigrid source
+        self._mg_followers = weakref.WeakSet()
 
     # ------------------------------------------------------------------ plumbing
     def close(self):
@@ -206,6 +209,11 @@ class FEM3D:
         destroys them on the next ``close()`` / context creation of the
         process, never from the collector's thread."""
         ctx = getattr(self, "_ctx", None)
+        for f in list(getattr(self, "_mg_followers", ())):
+            f.share_multigrid(None)            # no context keeps reading this one's hierarchy
+        if getattr(self, "_mg_src", None) is not None:
+            self._mg_src._mg_followers.discard(self)
+            self._mg_src = None
         self._ctx = None
         if ctx is not None and ctx.value:
             _lib.destroy(ctx)
@@ -429,6 +437,29 @@ class FEM3D:
             if k in self.__dict__:
                 setattr(t, k, getattr(self, k))
         return t
+
+    def share_multigrid(self, src: "FEM3D | None"):
+        """Take the per-density multigrid operators of ``src`` (a context of the
+        same model) by a device copy instead of computing them
+        (vox8_share_multigrid).  The caller gives both the same densities and
+        runs ``src.prepare_multigrid`` before each solve of this context; a
+        stale source only weakens the preconditioner, never the answer.
+        ``None`` ends the sharing."""
+        if self._ctx is None:
+            return
+        call("vox8_share_multigrid", self._ctx, None if src is None else src._ctx)
+        if self._mg_src is not None:
+            self._mg_src._mg_followers.discard(self)
+        self._mg_src = src
+        if src is not None:
+            src._mg_followers.add(self)
+
+    def prepare_multigrid(self, rho_active, ncols: int = 1):
+        """Set the density and bring the multigrid hierarchy up to date for it
+        (with work space for ``ncols`` columns); returns when the device work
+        is done."""
+        self._set_density(rho_active)
+        self._call("vox8_prepare_multigrid", int(ncols))
 
     def set_mg_galerkin(self, enable: bool = True):
         """Coarse levels of this context's multigrid preconditioner: Galerkin

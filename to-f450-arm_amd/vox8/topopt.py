@@ -239,6 +239,11 @@ class SIMPOptimizer:
             self._modal_stream = torch.cuda.Stream(device=dev)
             with torch.cuda.stream(self._modal_stream):
                 self._fem_modal = fem.twin()
+            if fem.precision == "mixed":
+                # both solves use one density per iteration: the modal context
+                # copies the static context's multigrid operators (prepared
+                # before the hand-over in step()) instead of building its own
+                self._fem_modal.share_multigrid(fem)
             self._pool = ThreadPoolExecutor(max_workers=1)
             self._overlap = False
             # one pair of events for the whole run: the solves' hand-over
@@ -280,6 +285,8 @@ class SIMPOptimizer:
             # never records an event on another thread's stream (which could
             # land inside a graph capture that thread has open).
             ready, done = self._ev_ready, self._ev_done
+            if self._fem_modal._mg_src is fem:
+                fem.prepare_multigrid(rho_phys, ncols=self.F.shape[0])
             ready.record()
             X0 = self.X
 
