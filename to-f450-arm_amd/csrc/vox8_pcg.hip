@@ -10,6 +10,10 @@
 // one-solve-per-load-case loop, src/topopt.py:287-290), but all columns share
 // every launch: the element factors and stencil are read once per iteration
 // for all load cases.  All inner products and norms accumulate in FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "vox8_kernels.h"
 #include "vox8_pcg.h"
 
@@ -303,7 +307,16 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
   fuse.scal = w.scal;
   fuse.partials = w.partials;
   fuse.counters = w.counters;
+  // Iterations are launched in chunks between host reads of the column
+  // states.  Iterations launched past a column's convergence are no-ops on
+  // the device, but each still costs its launches (a multigrid cycle is ~150
+  // of them), so after the first reads the next chunk is the remaining
+  // iteration count extrapolated from the slowest column's observed
+  // residual reduction per iteration (plus one), doubling only while no
+  // rate is known yet.
   int launched = 0, chunk = chunk0;
+  std::vector<double> rr_seen(nrhs, -1.0);
+  std::vector<int> it_seen(nrhs, 0);
   while (launched < maxiter) {
     const int todo = chunk < maxiter - launched ? chunk : maxiter - launched;
     for (int it = 0; it < todo; ++it) {
@@ -335,10 +348,27 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
     VOX8_HIP(hipMemcpyAsync(w.host_scal, w.scal, sizeof(PcgScalars) * nrhs, hipMemcpyDeviceToHost, st));
     VOX8_HIP(hipStreamSynchronize(st));
     bool all = true;
-    for (int j = 0; j < nrhs; ++j)
-      if (!w.host_scal[j].done && w.host_scal[j].iters < maxiter) all = false;
+    double need = 0.0;              // iterations the slowest running column still needs
+    bool known = true;
+    for (int j = 0; j < nrhs; ++j) {
+      const PcgScalars& sc = w.host_scal[j];
+      if (sc.done || sc.iters >= maxiter) continue;
+      all = false;
+      const double rr = sc.rr;
+      if (rr_seen[j] > 0.0 && rr > 0.0 && sc.iters > it_seen[j] && rr < rr_seen[j] && sc.atol2 > 0.0) {
+        const double per_it = std::log(rr / rr_seen[j]) / (sc.iters - it_seen[j]);   // < 0
+        need = std::max(need, std::log(sc.atol2 / rr) / per_it);
+      } else {
+        known = false;
+      }
+      rr_seen[j] = rr;
+      it_seen[j] = sc.iters;
+    }
     if (all) break;
-    chunk = chunk * 2 < 256 ? chunk * 2 : 256;
+    if (known)
+      chunk = need < 255.0 ? std::max(1, (int)std::ceil(need) + 1) : 256;
+    else
+      chunk = chunk * 2 < 256 ? chunk * 2 : 256;
   }
   // As in SciPy, a column that reaches maxiter is reported unconverged even if
   // its last step met the tolerance (the stop test runs before an iteration).

@@ -635,9 +635,9 @@ static int fast_blocks(const Grid& g, int ty, int zc) {
 // step (the multigrid's kernel) of each candidate shape on the caller's work
 // vectors and keeps the fastest.  An FP32 result does not depend on the shape (every node's sum
 // is formed in the same order whatever the tiling), so tuning changes no
-// value.  FP64 launches (PCG and LOBPCG products, whose fused dot products
-// are summed per workgroup) keep the fixed rule above, which keeps those
-// results independent of timing.
+// value.  FP64 launches are measured the same way on first use, except the
+// PCG's, whose fused dot products are summed per workgroup: they keep the
+// fixed rule above, which keeps those results independent of timing.
 struct ShapeKey {
   int nx, ny, nz, ncols;
   bool operator<(const ShapeKey& o) const {
@@ -775,6 +775,106 @@ int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, cons
   return OK;
 }
 
+// ---- FP64 launch shapes ------------------------------------------------------
+// A launch without fused dot products gives bitwise the same values for every
+// z chunk, so its chunk is measured on first use (per grid, columns and
+// operator), like the FP32 shapes.  A PCG launch sums its dot products per
+// workgroup, so its chunk must not depend on timing: it keeps the rule above.
+// the distinct z chunkings of a grid (chunks of ceil(NZ / n) planes)
+static std::vector<int> zchunk_candidates(const Grid& g, int ty, int ncols) {
+  std::vector<int> zcs;
+  for (int chunks = 1; chunks <= g.NZ && (int)zcs.size() < 12; ++chunks) {
+    const int z = (g.NZ + chunks - 1) / chunks;
+    if ((g.NZ + z - 1) / z != chunks) continue;
+    const int64_t wgs = (int64_t)tiles_x(g) * tiles_y(g, ty) * chunks * ncols;
+    if (wgs < 64 && z > 1) continue;
+    if (wgs > 16384) break;
+    zcs.push_back(z);
+  }
+  if (zcs.empty()) zcs.push_back(g.NZ);
+  return zcs;
+}
+
+struct Shape64Key {
+  int nx, ny, nz, ncols, op;
+  bool operator<(const Shape64Key& o) const {
+    if (nx != o.nx) return nx < o.nx;
+    if (ny != o.ny) return ny < o.ny;
+    if (nz != o.nz) return nz < o.nz;
+    if (ncols != o.ncols) return ncols < o.ncols;
+    return op < o.op;
+  }
+};
+static std::map<Shape64Key, int> g_zc64;
+
+// The measured chunk of an unfused FP64 launch: on the first launch of a key
+// every candidate is timed (one warm-up, three timed launches) writing the
+// caller's outputs, which the real launch then overwrites.  Under stream
+// capture (no host waits allowed) the fixed rule stands in, unrecorded.
+template <typename Launch>
+static int tuned_zchunk64(const Grid& g, int op, int ncols, hipStream_t st, Launch& launch,
+                          int* zc_out) {
+  const Shape64Key key{g.nx, g.ny, g.nz, ncols, op};
+  {
+    std::lock_guard<std::mutex> lock(g_shape_mutex);
+    const auto it = g_zc64.find(key);
+    if (it != g_zc64.end()) { *zc_out = it->second; return OK; }
+  }
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  VOX8_HIP(hipStreamIsCapturing(st, &cs));
+  const int rule = fast_zchunk(g, 8, ncols);
+  if (cs != hipStreamCaptureStatusNone) { *zc_out = rule; return OK; }
+  HipSerial hip_lock(hip_serial());     // event creation and host waits
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  VOX8_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("hipEventCreate failed");
+    return ERR_HIP;
+  }
+  struct Cand { int zc; float ms; };
+  std::vector<Cand> done;
+  int rc = OK;
+  for (int z : zchunk_candidates(g, 8, ncols)) {
+    launch(8, z);                                        // warm-up
+    if (hipEventRecord(e0, st) != hipSuccess) { rc = ERR_HIP; break; }
+    for (int i = 0; i < 3; ++i) launch(8, z);
+    float ms = 0.f;
+    if (hipEventRecord(e1, st) != hipSuccess || hipGetLastError() != hipSuccess ||
+        hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = ERR_HIP;
+      break;
+    }
+    done.push_back(Cand{z, ms / 3});
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != OK) {
+    set_error("FP64 stencil shape timing failed");
+    return rc;
+  }
+  // within 3 % of the fastest counts as equal; the least halo work then wins
+  float fastest = 1e30f;
+  for (const Cand& c : done) fastest = std::min(fastest, c.ms);
+  int best = rule;
+  int64_t best_work = INT64_MAX;
+  for (const Cand& c : done) {
+    if (c.ms > 1.03f * fastest) continue;
+    const int64_t work = (int64_t)fast_blocks(g, 8, c.zc) * (c.zc + 1);
+    if (work < best_work || (work == best_work && c.zc > best)) { best_work = work; best = c.zc; }
+  }
+  static const bool log = env_int("VOX8_TUNE_LOG", 0) != 0;
+  if (log) {
+    fprintf(stderr, "[vox8 tune64] %dx%dx%d x%d op %d:", g.nx, g.ny, g.nz, ncols, op);
+    for (const Cand& c : done) fprintf(stderr, " %d:%.1fus", c.zc, 1e3f * c.ms);
+    fprintf(stderr, " -> %d (rule %d)\n", best, rule);
+  }
+  std::lock_guard<std::mutex> lock(g_shape_mutex);
+  g_zc64[key] = best;
+  *zc_out = best;
+  return OK;
+}
+
 template <typename T>
 int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,
                       const T* fM, const uint8_t* fixed, const T* x, int64_t ldx, T* yK, T* yM,
@@ -789,9 +889,10 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     set_error("stencil epilogue modes apply to K alone");
     return ERR_ARG;
   }
-  const StencilShape shp = stencil_shape<T>(g, nrhs);
-  const int ty = shp.ty, zc = shp.zc;
-  const int nb = fast_blocks(g, ty, zc);   // workgroups per column
+  // one launch at a given shape (the kernel instance follows the pattern,
+  // operator, fusion, epilogue and tile rows)
+  auto launch = [&](int ty, int zc) {
+    const int nb = fast_blocks(g, ty, zc);   // workgroups per column
 #define VOX8_LAUNCH_TY(PAT, K, M, F, MD, TYV)                                                     \
   hipLaunchKernelGGL((k_apply_fast<T, PAT, K, M, F, MD, TYV>), dim3(nb * nrhs), dim3(TX * TYV), 0, \
                      st, g, zc, tiles_x(g), tiles_y(g, TYV), nrhs, C, fK, fM, fixed, x, ldx, yK, yM,\
@@ -815,12 +916,20 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     else VOX8_LAUNCH(PAT, true, false, false, EPI_PLAIN);                               \
   } else if (op == OP_M) VOX8_LAUNCH(PAT, false, true, false, EPI_PLAIN);                \
   else VOX8_LAUNCH(PAT, true, true, false, EPI_PLAIN);
-  if (pattern == PAT_CUBE) { VOX8_PAT(PAT_CUBE) }
-  else if (pattern == PAT_ORTHO) { VOX8_PAT(PAT_ORTHO) }
-  else { VOX8_PAT(PAT_BLOCK) }
+    if (pattern == PAT_CUBE) { VOX8_PAT(PAT_CUBE) }
+    else if (pattern == PAT_ORTHO) { VOX8_PAT(PAT_ORTHO) }
+    else { VOX8_PAT(PAT_BLOCK) }
 #undef VOX8_PAT
 #undef VOX8_LAUNCH
 #undef VOX8_LAUNCH_TY
+    return nb;
+  };
+  StencilShape shp = stencil_shape<T>(g, nrhs);
+  if constexpr (sizeof(T) == 8) {
+    if (!fz && env_int("VOX8_ZCHUNK", 0) <= 0)
+      VOX8_TRY(tuned_zchunk64(g, op, nrhs, st, launch, &shp.zc));
+  }
+  const int nb = launch(shp.ty, shp.zc);
   VOX8_HIP(hipGetLastError());
   if (fz) VOX8_TRY(launch_pq_finish(fuse, nb, nrhs, st));
   return OK;
