@@ -740,7 +740,7 @@ int vox8_sensitivities(vox8_ctx* c, const double* d_U, int nrhs, int64_t ld, con
   VOX8_TRY(launch_quadforms(c->g, c->fK, c->fM, c->d_active, c->dKe, c->dMe, d_U, ld, nrhs,
                             c->d_wts, d_phi, nrhs > 0 ? ce : nullptr, c->d_kphi, c->d_mphi,
                             c->red_partials, c->red_counters, c->d_sums, c->d_tip, c->n_tip,
-                            c->tip_m, c->st));
+                            c->tip_m, c->st, nullptr, c->pattern != PAT_DENSE ? &c->c64 : nullptr));
   if (d_dc || d_dlam)
     VOX8_TRY(launch_sens_finish(c->g.n_elem, c->d_active, d_rho, p, e_min, mthr, mq, ce,
                                 nrhs > 0 ? d_dc : nullptr, c->d_kphi, c->d_mphi, lam,
@@ -913,7 +913,7 @@ int vox8_simp_tail(vox8_ctx* c, const vox8_tail_args* a, double* h_state) {
     VOX8_TRY(launch_quadforms(c->g, c->fK, c->fM, c->d_active, c->dKe, c->dMe, nullptr, nd, 0,
                               c->d_wts, a->d_vprev, nullptr, c->d_kphi, c->d_mphi, c->red_partials,
                               c->red_counters, c->d_sums, c->d_tip, c->n_tip, c->tip_m, c->st,
-                              gate));
+                              gate, c->pattern != PAT_DENSE ? &c->c64 : nullptr));
     VOX8_TRY(launch_sens_finish(ne, c->d_active, a->d_rho, a->p, a->e_min, a->mthr, a->mq, c->d_ce,
                                 nullptr, c->d_kphi, c->d_mphi, 0.0, c->d_sums, a->d_dlam, c->st,
                                 state + 1, gate));

@@ -2,7 +2,7 @@
 // `gate` (optional, device): the launch does nothing unless *gate > 0, so a
 // launch sequence can skip a branch decided on the device without a host read.
 #pragma once
-#include "vox8_kernels.h"
+#include "vox8_fast.h"
 
 namespace vox8 {
 enum FilterMode { FILTER_FWD = 0, FILTER_ADJ = 1, FILTER_DEN = 2 };
@@ -14,7 +14,7 @@ int launch_quadforms(const Grid& g, const double* fK, const double* fM, const ui
                      const double* wts, const double* phi, double* ce, double* kphi,
                      double* mphi, double* partials, unsigned int* counter, double* sums,
                      const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st,
-                     const double* gate = nullptr);
+                     const double* gate = nullptr, const FastCoef<double>* mono = nullptr);
 int launch_sens_finish(int64_t n, const uint8_t* active, const double* rho, double p, double emin,
                        double mthr, double mq, const double* ce, double* dc, const double* kphi,
                        const double* mphi, double lam, const double* vmv, double* dlam,

@@ -196,16 +196,158 @@ __global__ void k_sens_finish(int64_t n, const uint8_t* __restrict__ active,
   }
 }
 
+// The same forms for a box element whose Ke / Me the structured stencil
+// handles (pattern != dense): with m = H^T v the butterfly coefficients of the
+// element's corner values v (per component, monomial order of the stencil),
+// v^T Ke v = m^T Kt m over the three diagonal blocks of Kt (lin 9x9, bil 9x9,
+// tri 3x3 in C, scaled so that no other factor enters) and v^T Me v =
+// sum_k md[k] sum_c m_c[k]^2: about 240 flops per element and vector instead
+// of 1152.  Block sums go to partials[block][nv] and k_quadforms_finish adds
+// them in a fixed order (no grid-wide fence, bitwise reproducible).
+__device__ __forceinline__ void qf_mono(const double* __restrict__ v, int64_t ld,
+                                        const int64_t dof[8], double m[3][8]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double w[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) w[b] = v[dof[b] + c];
+    // corners in the reference's local order -> x + 2y + 4z order of the butterflies
+    const double x0 = w[0], x1 = w[1], x2 = w[3], x3 = w[2], x4 = w[4], x5 = w[5], x6 = w[7], x7 = w[6];
+    const double s0 = x1 + x0, d0 = x1 - x0, s1 = x3 + x2, d1 = x3 - x2;
+    const double s2 = x5 + x4, d2 = x5 - x4, s3 = x7 + x6, d3 = x7 - x6;
+    const double ss0 = s1 + s0, sd0 = s1 - s0, ds0 = d1 + d0, dd0 = d1 - d0;
+    const double ss1 = s3 + s2, sd1 = s3 - s2, ds1 = d3 + d2, dd1 = d3 - d2;
+    m[c][0] = ss1 + ss0; m[c][1] = ds1 + ds0; m[c][2] = sd1 + sd0; m[c][3] = ss1 - ss0;
+    m[c][4] = dd1 + dd0; m[c][5] = sd1 - sd0; m[c][6] = ds1 - ds0; m[c][7] = dd1 - dd0;
+  }
+  (void)ld;
+}
+
+__device__ __forceinline__ double qf_stiff(const FastCoef<double>& C, const double m[3][8]) {
+  double al[9], ab[9], at[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    al[c] = m[c][1]; al[3 + c] = m[c][2]; al[6 + c] = m[c][3];
+    ab[c] = m[c][4]; ab[3 + c] = m[c][5]; ab[6 + c] = m[c][6];
+    at[c] = m[c][7];
+  }
+  double q = 0.0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    double tl = 0.0, tb = 0.0;
+#pragma unroll
+    for (int s = 0; s < 9; ++s) { tl += C.lin[r][s] * al[s]; tb += C.bil[r][s] * ab[s]; }
+    q += al[r] * tl + ab[r] * tb;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) q += at[r] * (C.tri[r][0] * at[0] + C.tri[r][1] * at[1] + C.tri[r][2] * at[2]);
+  return q;
+}
+
+__global__ void __launch_bounds__(256)
+k_quadforms_mono(Grid g, FastCoef<double> C, const double* __restrict__ fK,
+                 const double* __restrict__ fM, const uint8_t* __restrict__ active,
+                 const double* __restrict__ U, int64_t ldu, int nrhs, const double* __restrict__ wts,
+                 const double* __restrict__ phi, double* __restrict__ ce, double* __restrict__ kphi,
+                 double* __restrict__ mphi, double* __restrict__ partials,
+                 const double* __restrict__ gate) {
+  if (gate && !(*gate > 0.0)) return;
+  __shared__ double sh[8];
+  double acc[17];
+#pragma unroll
+  for (int v = 0; v < 17; ++v) acc[v] = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < g.n_elem;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (!active[e]) {
+      if (ce) ce[e] = 0.0;
+      if (phi) { kphi[e] = 0.0; mphi[e] = 0.0; }
+      continue;
+    }
+    const int ex = (int)(e % g.nx), ey = (int)((e / g.nx) % g.ny),
+              ez = (int)(e / ((int64_t)g.nx * g.ny));
+    int64_t dof[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      dof[b] = 3 * ((ex + local_off(b, 0)) +
+                    (int64_t)g.NX * ((ey + local_off(b, 1)) + (int64_t)g.NY * (ez + local_off(b, 2))));
+    double cw = 0.0;
+    const double fk = fK[e];
+#pragma unroll 1
+    for (int j = 0; j < nrhs; ++j) {
+      double m[3][8];
+      qf_mono(U + j * ldu, ldu, dof, m);
+      const double q = qf_stiff(C, m);
+      cw += wts[j] * q;
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+        if (v == j) acc[v] += fk * q;            // unrolled: acc stays in registers
+    }
+    if (ce) ce[e] = cw;
+    if (phi) {
+      double m[3][8];
+      qf_mono(phi, 0, dof, m);
+      const double qk = qf_stiff(C, m);
+      double qm = 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) qm += C.md[k] * (m[0][k] * m[0][k] + m[1][k] * m[1][k] + m[2][k] * m[2][k]);
+      kphi[e] = qk;
+      mphi[e] = qm;
+      acc[16] += fM[e] * qm;
+    }
+  }
+  // block sums of the nrhs compliance values and of phi^T M phi (slot 16)
+#pragma unroll
+  for (int v = 0; v < 17; ++v) {
+    const double t = block_sum(acc[v], sh);
+    if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * 17 + v] = t;
+  }
+}
+
+// sums[j] (j < nrhs) and sums[nrhs] = phi^T M phi + the tip lump, from the
+// block partials in a fixed order; one block.
+__global__ void __launch_bounds__(256)
+k_quadforms_finish(int nb, int nrhs, bool have_phi, const double* __restrict__ partials,
+                   const double* __restrict__ phi, const int64_t* __restrict__ tip_nodes, int n_tip,
+                   double tip_m, double* __restrict__ sums, const double* __restrict__ gate) {
+  if (gate && !(*gate > 0.0)) return;
+  __shared__ double sh[8];
+  for (int v = 0; v <= nrhs; ++v) {
+    if (v == nrhs && !have_phi) break;
+    const int slot = v < nrhs ? v : 16;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) t += partials[(int64_t)b * 17 + slot];
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+      if (v == nrhs) {
+        // v^T M v includes the rho-independent tip lump (src/fem.py:511-520)
+        double tl = 0.0;
+        for (int i = 0; i < n_tip; ++i)
+          for (int c = 0; c < 3; ++c) tl += phi[3 * tip_nodes[i] + c] * phi[3 * tip_nodes[i] + c];
+        t += tip_m * tl;
+      }
+      sums[v] = t;
+    }
+  }
+}
+
 int launch_quadforms(const Grid& g, const double* fK, const double* fM, const uint8_t* active,
                      const double* Ke, const double* Me, const double* U, int64_t ldu, int nrhs,
                      const double* wts, const double* phi, double* ce, double* kphi,
                      double* mphi, double* partials, unsigned int* counter, double* sums,
                      const int64_t* tip_nodes, int n_tip, double tip_m, hipStream_t st,
-                     const double* gate) {
+                     const double* gate, const FastCoef<double>* mono) {
   VOX8_CHECK(nrhs >= 0 && nrhs <= 16, "quadforms: at most 16 load cases");
-  hipLaunchKernelGGL(k_quadforms, dim3(nblk(g.n_elem)), dim3(256), 0, st, g, fK, fM, active, Ke,
-                     Me, U, ldu, nrhs, wts, phi, ce, kphi, mphi, partials, counter, sums,
-                     tip_nodes, n_tip, tip_m, gate);
+  if (mono) {
+    const unsigned nb = nblk(g.n_elem, 2048);    // partials: nb x 17 doubles
+    hipLaunchKernelGGL(k_quadforms_mono, dim3(nb), dim3(256), 0, st, g, *mono, fK, fM, active, U, ldu,
+                       nrhs, wts, phi, ce, kphi, mphi, partials, gate);
+    hipLaunchKernelGGL(k_quadforms_finish, dim3(1), dim3(256), 0, st, (int)nb, nrhs, phi != nullptr,
+                       (const double*)partials, phi, tip_nodes, n_tip, tip_m, sums, gate);
+  } else {
+    hipLaunchKernelGGL(k_quadforms, dim3(nblk(g.n_elem)), dim3(256), 0, st, g, fK, fM, active, Ke,
+                       Me, U, ldu, nrhs, wts, phi, ce, kphi, mphi, partials, counter, sums,
+                       tip_nodes, n_tip, tip_m, gate);
+  }
   VOX8_HIP(hipGetLastError());
   return OK;
 }
