@@ -918,8 +918,12 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
                        L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b, skip);
     VOX8_HIP(hipGetLastError());
+    // the coarse level's last post-smoothing step is folded into the
+    // prolongation below it, except into the finest level: there the
+    // prolongation is a wide gather-bound pass, and reading the direction
+    // beside the iterate costs more than the separate add on the coarse grid
     const float* dpend = nullptr;
-    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip, nullptr, &dpend));
+    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip, nullptr, l > 0 ? &dpend : nullptr));
     const dim3 pgrid((unsigned)((L.g.n_nodes + 255) / 256), ncols);
     if (dpend)
       hipLaunchKernelGGL(k_mg_prolong_add<true>, pgrid, dim3(256), 0, st, Cn.g, L.g, (const float*)Cn.x,
