@@ -147,8 +147,9 @@ def kernel_rooflines(fem, geom, rho, nrhs=6, reps=30):
     alg = 16 * n + 8 * ne + nn
     out["k_fp64"] = dict(kernel="k_apply_fast<double,CUBE,K> (FP64 K product, 1 column)", t=t, alg=alg,
                          per_unit="16 B per DOF (read x; write Kx), + 8 B/element factor, 1 B/node mask")
-    # FP64 K product, 5 columns: the block PCG's matvec over the five load
-    # cases (its launch also sums p.Kp per workgroup)
+    # FP64 K product, 5 columns: the block PCG's matvec width over the five
+    # load cases (this launch is the unfused form; the PCG's own also sums
+    # p.Kp per workgroup, at the same z chunk)
     x5, y5 = x[:5].contiguous(), yk[:5].contiguous()
     t = _event_time(fem, lambda: fem._call("vox8_apply", _lib.OP_K, _lib.PREC_FP64, _lib.ptr(x5),
                                            _lib.ptr(y5), None, 5, n), reps)
