@@ -376,12 +376,17 @@ k_rayleigh_ritz(int m, int k, const double* __restrict__ G, double* __restrict__
 // One row per thread (no grid-stride loop): with a loop the compiler keeps
 // all of C in registers across iterations (M*K doubles: one wave per SIMD,
 // and scratch spills from k = 8 on); here C is read from LDS as it is used.
+// conv (may be null): pairs locked so far; their new direction P' is written
+// as zero (their P, K P and M P stay zero: the K and M products skip them).
 template <int K, int NB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB * K * K > 150 ? 2 : 4)))
-k_ritz_combine(int64_t n, double* __restrict__ S, const double* __restrict__ Cg) {
+k_ritz_combine(int64_t n, double* __restrict__ S, const double* __restrict__ Cg,
+               const int* __restrict__ conv) {
   constexpr int M = NB * K;
   __shared__ double C[M * K];
+  __shared__ int lk[K];
   for (int e = threadIdx.x; e < M * K; e += blockDim.x) C[e] = Cg[e];
+  if (threadIdx.x < K) lk[threadIdx.x] = conv ? conv[threadIdx.x] : 0;
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -396,21 +401,26 @@ k_ritz_combine(int64_t n, double* __restrict__ S, const double* __restrict__ Cg)
 #pragma unroll
     for (int i = K; i < M; ++i) p += v[i] * C[i * K + j];
     S[(int64_t)j * n + r] = x + p;
-    if (NB > 1) S[(int64_t)(2 * K + j) * n + r] = p;
+    if (NB > 1) S[(int64_t)(2 * K + j) * n + r] = lk[j] ? 0.0 : p;
   }
 }
 
 // R = AX - BX diag(lam) (exact products), ||R_j||^2 reduced, W = D^-1 R
-// (or W = R when a preconditioner follows).
+// (or W = R when a preconditioner follows); W = 0 for the pairs in conv.
 template <int K>
 __global__ void __launch_bounds__(256)
 k_residual(int64_t n, double* __restrict__ S, const double* __restrict__ AS,
            const double* __restrict__ BS, const double* __restrict__ lamg,
            const double* __restrict__ dinv, double* __restrict__ partials,
-           unsigned int* __restrict__ counter, double* __restrict__ res_out) {
+           unsigned int* __restrict__ counter, double* __restrict__ res_out,
+           const int* __restrict__ conv) {
   __shared__ double lam[K], sh[8], bs[K];
+  __shared__ int lk[K];
   const int tid = threadIdx.x;
-  if (tid < K) lam[tid] = lamg[tid];
+  if (tid < K) {
+    lam[tid] = lamg[tid];
+    lk[tid] = conv ? conv[tid] : 0;
+  }
   __syncthreads();
   double nrm[K];
 #pragma unroll
@@ -422,7 +432,7 @@ k_residual(int64_t n, double* __restrict__ S, const double* __restrict__ AS,
     for (int j = 0; j < K; ++j) {
       const double res = AS[(int64_t)j * n + r] - lam[j] * BS[(int64_t)j * n + r];
       nrm[j] += res * res;
-      S[(int64_t)(K + j) * n + r] = di * res;
+      S[(int64_t)(K + j) * n + r] = lk[j] ? 0.0 : di * res;   // locked: W stays zero
     }
   }
 #pragma unroll
@@ -482,9 +492,10 @@ static int gram_and_rr(LobpcgWork& w, int m, int k, hipStream_t st, double drop_
 template <int K>
 static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStream_t st) {
   const dim3 g((unsigned)((w.n + 255) / 256)), b(256);
-  if (nb == 1) hipLaunchKernelGGL((k_ritz_combine<K, 1>), g, b, 0, st, w.n, w.S, C);
-  else if (nb == 2) hipLaunchKernelGGL((k_ritz_combine<K, 2>), g, b, 0, st, w.n, w.S, C);
-  else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, C);
+  const int* cv = w.conv;
+  if (nb == 1) hipLaunchKernelGGL((k_ritz_combine<K, 1>), g, b, 0, st, w.n, w.S, C, cv);
+  else if (nb == 2) hipLaunchKernelGGL((k_ritz_combine<K, 2>), g, b, 0, st, w.n, w.S, C, cv);
+  else hipLaunchKernelGGL((k_ritz_combine<K, 3>), g, b, 0, st, w.n, w.S, C, cv);
 }
 
 // Soft locking: a pair whose residual norm fell below tol (sticky, as the
@@ -494,17 +505,26 @@ static void launch_combine(const LobpcgWork& w, int nb, const double* C, hipStre
 // X column keeps being refined by the other directions.  Without this the
 // noise-level W of converged pairs, normalised by the K scaling, pollutes the
 // basis and can stall the remaining pairs.
+// Pairs locked in earlier iterations need nothing here: the residual and
+// Ritz-update kernels write their W and P as zero and the products skip
+// them.  Only a pair whose residual fell below tol in this iteration has its
+// six columns zeroed; in an iteration with none the kernel reads the flags and
+// returns.  The flags themselves are updated by k_lock_flags afterwards, so
+// every block of this kernel sees the same old flags.
 template <int K>
 __global__ void __launch_bounds__(256)
 k_lock(int64_t n, double* __restrict__ S, double* __restrict__ AS, double* __restrict__ BS,
-       const double* __restrict__ res2, int* __restrict__ conv, double tol2) {
-  __shared__ int lk[K];
+       const double* __restrict__ res2, const int* __restrict__ conv, double tol2) {
+  __shared__ int lk[K], any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
   if (threadIdx.x < K) {
     const int j = threadIdx.x;
-    lk[j] = conv[j] || fabs(res2[j]) < tol2;
-    if (blockIdx.x == 0) conv[j] = lk[j];
+    lk[j] = !conv[j] && fabs(res2[j]) < tol2;             // newly converged
+    if (lk[j]) any = 1;
   }
   __syncthreads();
+  if (!any) return;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x) {
 #pragma unroll
@@ -520,10 +540,17 @@ k_lock(int64_t n, double* __restrict__ S, double* __restrict__ AS, double* __res
   }
 }
 
+// conv[j] |= (residual below tol): after k_lock, in stream order.
+__global__ void k_lock_flags(int k, const double* __restrict__ res2, int* __restrict__ conv, double tol2) {
+  const int j = threadIdx.x;
+  if (j < k && fabs(res2[j]) < tol2) conv[j] = 1;
+}
+
 template <int K>
 static void launch_lock(const LobpcgWork& w, double tol2, hipStream_t st) {
   hipLaunchKernelGGL((k_lock<K>), dim3(grid_blocks(w.n)), dim3(256), 0, st, w.n, w.S, w.AS, w.BS,
-                     (const double*)w.res, w.conv, tol2);
+                     (const double*)w.res, (const int*)w.conv, tol2);
+  hipLaunchKernelGGL(k_lock_flags, dim3(1), dim3(64), 0, st, K, (const double*)w.res, w.conv, tol2);
 }
 
 __global__ void k_zero_int(int n, int* __restrict__ a) {
@@ -535,7 +562,7 @@ static void launch_residual(const LobpcgWork& w, const double* lam, const double
   const unsigned nb = grid_blocks(w.n);
   hipLaunchKernelGGL((k_residual<K>), dim3(nb), dim3(256), 0, st, w.n, w.S,
                      (const double*)w.AS, (const double*)w.BS, lam, dinv, w.res_partials, w.counters,
-                     w.res);
+                     w.res, (const int*)w.conv);
   hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(256), 0, st, K, (int)nb,
                      (const double*)w.res_partials, w.res);
 }
