@@ -7,7 +7,8 @@ levels coarsened 2:1 (ceil), factor = mean of 8 children, element matrix
 2^l Ke, trilinear P (masked by fixed DOFs on both sides; a coarse DOF
 is fixed when any fine node of its 3x3x3 interpolation stencil has it fixed), R = P^T, Chebyshev
 smoothing of D^-1 K on [lam_max/alpha, lam_max] (nu steps, nu_coarse from
-level nu_coarse_level on), dense coarsest solve; cycle
+level nu_coarse_level on; the three-term recurrence on the iterates, as the
+device runs it), dense coarsest solve; cycle
 index gamma (1 = V, 2 = W with a fresh residual after each coarse correction).
 galerkin=True replaces the rediscretised coarse operators by P^T K P (sparse
 matrix products), the device's Galerkin option (vox8_set_mg_galerkin).
@@ -125,37 +126,39 @@ class VCycle:
         Kc[~C["free"], ~C["free"]] = 1.0
         C["inv"] = np.linalg.inv(Kc)
 
-    def smooth(self, L, x, r, d, nu):
+    def smooth(self, L, x, b, nu):
+        """nu Chebyshev steps of the first kind on [lam_max/alpha, lam_max] in
+        the three-term form on the iterates (the device's form):
+        x_1 = x_0 + D^-1 (b - K x_0) / theta, then
+        x_{k+1} = x_k + rho_k rho_{k-1} (x_k - x_{k-1}) + 2 rho_k / delta D^-1 (b - K x_k)."""
         lmax = L["lam_max"]
         lmin = lmax / self.alpha
         theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
         sigma = theta / delta
+        dinv, K = L["dinv"], L["K"]
+        xprev, x = x, x + dinv * (b - K @ x) / theta
         rho = 1.0 / sigma
-        for _ in range(nu):
+        for _ in range(nu - 1):
             rho1 = 1.0 / (2 * sigma - rho)
-            x = x + d
-            r = r - L["K"] @ d
-            d = rho1 * rho * d + 2 * rho1 / delta * L["dinv"] * r
+            xprev, x = x, x + rho1 * rho * (x - xprev) + 2 * rho1 / delta * dinv * (b - K @ x)
             rho = rho1
-        return x, r
+        return x
 
     def __call__(self, b, l=0):
         """Cycle of index gamma: pre-smoothing from zero, gamma coarse-grid
-        corrections each followed by a fresh fine residual (one when the next
-        level is the exact coarsest solve or this level has fewer than
-        w_min_dofs DOFs), post-smoothing."""
+        corrections each on a fresh fine residual (one when the next level is
+        the exact coarsest solve or this level has fewer than w_min_dofs
+        DOFs), post-smoothing."""
         L = self.levels[l]
         if l == len(self.levels) - 1:
             return np.where(L["free"], L["inv"] @ b, 0.0)
         nu = self.nu_coarse if (self.nu_coarse > 0 and l >= self.nu_coarse_level) else self.nu
-        theta = 0.5 * (L["lam_max"] + L["lam_max"] / self.alpha)
-        x, r = self.smooth(L, np.zeros_like(b), b.copy(), L["dinv"] * b / theta, nu)
+        x = self.smooth(L, np.zeros_like(b), b, nu)
         g = self.gamma if (l + 1 < len(self.levels) - 1 and L["K"].shape[0] >= self.w_min_dofs) else 1
         for _ in range(g):
-            x = x + L["P"] @ self(L["P"].T @ r, l + 1)
             r = b - L["K"] @ x
-        x, _ = self.smooth(L, x, r, L["dinv"] * r / theta, nu)
-        return x
+            x = x + L["P"] @ self(L["P"].T @ r, l + 1)
+        return self.smooth(L, x, b, nu)
 
 
 def hash_start(n):

@@ -43,20 +43,32 @@ int fast_coefficients(const double* Ke, const double* Me, FastCoef<double>* c64,
 
 // EPI_CHEB0: the first Chebyshev step from a zero iterate, x = 0, r = b:
 // the input vector is b and the direction d = c0 D^-1 b is formed on load.
-enum EpiMode : int { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB = 2, EPI_CHEB0 = 3 };
+// EPI_X0 / EPI_X: Chebyshev steps in the three-term form on the iterates
+// alone (no residual or direction vectors),
+//   x_{k+1} = x_k + c1 (x_k - x_{k-1}) + c2 D^-1 (b - A x_k),
+// with the input vector x_k.  EPI_X0 is the step from x_1 = c0 D^-1 b with
+// x_0 = 0: the input is b and x_1 is formed on load.  EPI_X takes x_{k-1}
+// from `xprev` (prev 1), as c0 D^-1 b (prev 2: the step after EPI_X0), or
+// drops the c1 term (prev 0: the first step from a non-zero iterate).
+enum EpiMode : int { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB = 2, EPI_CHEB0 = 3, EPI_X0 = 4,
+                     EPI_X = 5 };
 
 // Node-local epilogue of a K product (multigrid smoothing), vectors with ld.
 template <typename T>
 struct Epilogue {
   int mode = EPI_PLAIN;
-  const T* b = nullptr;      // RESID: right-hand side
+  const T* b = nullptr;      // RESID, X: right-hand side
   T* r = nullptr;            // RESID: out; CHEB: in/out residual
-  T* xs = nullptr;           // CHEB: iterate (+= d)
-  T* dnew = nullptr;         // out: next direction
+  T* xs = nullptr;           // CHEB: iterate (+= d); X0, X: out, x_{k+1}
+  T* dnew = nullptr;         // out: next direction (RESID: may be null)
   const T* dinv = nullptr;   // Jacobi inverse diagonal (one vector, no ld)
   T c0 = 0, c1 = 0, c2 = 0;
   int64_t ld = 0;
   const int* skip = nullptr;  // per column: non-zero = leave this column untouched
+  const T* xprev = nullptr;  // EPI_X, prev 1: x_{k-1}
+  int prev = 0;              // EPI_X: where x_{k-1} comes from (see EpiMode)
+  double* out64 = nullptr;   // EPI_X (FP32 structured stencil): x_{k+1} stored as FP64 here
+                             // instead of into xs
 };
 
 

@@ -12,8 +12,9 @@
 //     the fixed DOFs of their level (a coarse DOF is fixed when a fine node of
 //     its interpolation stencil has it fixed);
 //   * smoother: Chebyshev polynomial of D^-1 K on [lam_max/alpha, lam_max]
-//     (three-term recurrence, Saad, Iterative Methods, sec. 12.1), nu steps
-//     before and after; lam_max from Lanczos in the D inner product, +10 % margin;
+//     (three-term recurrence on the iterates, Saad, Iterative Methods,
+//     sec. 12.1), nu steps before and after; lam_max from Lanczos in the D
+//     inner product, +10 % margin;
 //   * coarsest level: dense inverse (in-place Gauss-Jordan of the SPD matrix).
 // Smoothing and transfers run in FP32 on the structured stencil with fused
 // Chebyshev epilogues; the outer Krylov method stays FP64.
@@ -128,19 +129,15 @@ __global__ void k_mg_restrict(Grid gf, Grid gc, const float* __restrict__ rf,
 
 // fine x += P coarse x (trilinear), fixed fine DOFs untouched.  Branch-free:
 // on an axis where the fine node coincides with a coarse one the second coarse
-// index equals the first and its term is not added.  ADD_D: the coarse
-// level's post-smoothing left its last step xc += dc to this kernel, which
-// reads xc + dc (the same FP32 sum the separate add would have stored).
-template <bool ADD_D>
+// index equals the first and its term is not added.
 __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
-                                 const float* __restrict__ dc, const uint8_t* __restrict__ fixf,
-                                 float* __restrict__ xf, const int* __restrict__ skip) {
+                                 const uint8_t* __restrict__ fixf, float* __restrict__ xf,
+                                 const int* __restrict__ skip) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= gf.n_nodes) return;
   const int col = blockIdx.y;
   if (skip && skip[col]) return;
   xc += (int64_t)col * gc.n_dofs;
-  if (ADD_D) dc += (int64_t)col * gc.n_dofs;
   xf += (int64_t)col * gf.n_dofs;
   int i, j, k;
   node_ijk(gf, n, i, j, k);
@@ -155,11 +152,7 @@ __global__ void k_mg_prolong_add(Grid gc, Grid gf, const float* __restrict__ xc,
       for (int cx = 0; cx < 2; ++cx) {
         const bool use = (cz == 0 || z0 != z1) && (cy == 0 || y0 != y1) && (cx == 0 || x0 != x1);
         const int64_t c = (cx ? x1 : x0) + (int64_t)gc.NX * ((cy ? y1 : y0) + (int64_t)gc.NY * (cz ? z1 : z0));
-        F3 v = ld_f3(xc + 3 * c);
-        if (ADD_D) {
-          const F3 d = ld_f3(dc + 3 * c);
-          v.x += d.x; v.y += d.y; v.z += d.z;
-        }
+        const F3 v = ld_f3(xc + 3 * c);
         s0 += use ? v.x : 0.0f;
         s1 += use ? v.y : 0.0f;
         s2 += use ? v.z : 0.0f;
@@ -380,30 +373,6 @@ __global__ void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __r
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     b[off + i] = (double)a[off + i];
-}
-
-// xs += d per column: the last Chebyshev step of a post-smoothing sweep (its
-// residual and next direction are never read, so the operator is not
-// applied).  d is zero at fixed DOFs, as every smoothing step leaves it.
-__global__ void k_mg_add_cols(int64_t n, const float* __restrict__ d, float* __restrict__ xs,
-                              const int* __restrict__ skip) {
-  if (skip && skip[blockIdx.y]) return;
-  const int64_t off = (int64_t)blockIdx.y * n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    xs[off + i] += d[off + i];
-}
-
-// The same step on the finest level, where the cycle's result leaves as FP64:
-// z = (double)(xs + d), the FP32 sum widened (the bits the separate add and
-// conversion give), without writing xs back.
-__global__ void k_mg_add_cols_out64(int64_t n, const float* __restrict__ d, const float* __restrict__ xs,
-                                    double* __restrict__ z, const int* __restrict__ skip) {
-  if (skip && skip[blockIdx.y]) return;
-  const int64_t off = (int64_t)blockIdx.y * n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    z[off + i] = (double)(xs[off + i] + d[off + i]);
 }
 
 // lambda_max(D^-1 K) by Lanczos in the D inner product <a, b>_D = sum a_i b_i / dinv_i,
@@ -839,63 +808,113 @@ struct ChebCoef { double theta, delta, sigma; };
 // pending (coarse levels): the post-smoothing sweep's last step xs += d is
 // not launched; *pending receives d, and the prolongation that reads xs adds
 // it on the fly.
-static int smooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st, bool from_zero,
-                  const int* skip, double* out64 = nullptr, const float** pending = nullptr) {
-  const double lmax = L.lam_max, lmin = lmax / alpha;
-  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+// Chebyshev smoothing of the first kind on [lam_max / alpha, lam_max] in the
+// three-term form on the iterates (EPI_X0 / EPI_X): with
+// theta = (lmax + lmin) / 2, delta = (lmax - lmin) / 2, sigma = theta / delta,
+// rho_0 = 1 / sigma and rho_k = 1 / (2 sigma - rho_{k-1}),
+//   x_1     = x_0 + D^-1 (b - A x_0) / theta,
+//   x_{k+1} = x_k + rho_k rho_{k-1} (x_k - x_{k-1}) + (2 rho_k / delta) D^-1 (b - A x_k).
+// A step reads x_k (the product's input), x_{k-1} and b and writes x_{k+1}:
+// no residual or direction vectors are kept.  The level's buffers x, d and
+// d2 hold three consecutive iterates in turn.
+struct ChebStep {
+  double c1, c2;
+};
+
+static Epilogue<float> step_epilogue(const MGLevel& L, int mode, double theta, ChebStep cs,
+                                     const int* skip) {
+  Epilogue<float> ep;
+  ep.mode = mode;
+  ep.b = L.b;
+  ep.dinv = L.dinv;
+  ep.c0 = (float)(1.0 / theta);
+  ep.c1 = (float)cs.c1;
+  ep.c2 = (float)cs.c2;
+  ep.ld = L.g.n_dofs;
+  ep.skip = skip;
+  return ep;
+}
+
+// The step coefficients (c1, c2) for k = 1 .. nu - 1 (index k - 1).
+static void cheb_steps(double lmax, double alpha, int nu, double* theta, std::vector<ChebStep>& cs) {
+  const double lmin = lmax / alpha;
+  const double th = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = th / delta;
   double rho = 1.0 / sigma;
-  const float* in = from_zero ? L.b : L.d;
-  float* out = from_zero ? L.d : L.d2;
-  for (int k = 0; k < nu; ++k) {
-    if (!from_zero && k == nu - 1) {
-      // the sweep's last step after a coarse correction: only xs += d is
-      // read afterwards (the cycle returns xs), so K d is not formed
-      if (pending) {
-        *pending = in;
-        break;
-      }
-      if (out64)
-        hipLaunchKernelGGL(k_mg_add_cols_out64, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st,
-                           L.g.n_dofs, in, (const float*)L.x, out64, skip);
-      else
-        hipLaunchKernelGGL(k_mg_add_cols, dim3(nb(L.g.n_dofs), ncols), dim3(256), 0, st, L.g.n_dofs,
-                           in, L.x, skip);
-      VOX8_HIP(hipGetLastError());
-      break;
-    }
+  cs.clear();
+  for (int k = 1; k < nu; ++k) {
     const double rho1 = 1.0 / (2.0 * sigma - rho);
-    Epilogue<float> ep;
-    ep.mode = (k == 0 && from_zero) ? EPI_CHEB0 : EPI_CHEB;
-    ep.r = L.r;
-    ep.xs = L.x;
-    ep.dnew = out;
-    ep.dinv = L.dinv;
-    ep.c0 = (float)(1.0 / theta);
-    ep.c1 = (float)(rho1 * rho);
-    ep.c2 = (float)(2.0 * rho1 / delta);
-    ep.ld = L.g.n_dofs;
-    ep.skip = skip;
-    VOX8_TRY(level_apply(L, in, nullptr, ncols, st, ep));
-    in = out;
-    out = (out == L.d) ? L.d2 : L.d;
+    cs.push_back({rho1 * rho, 2.0 * rho1 / delta});
     rho = rho1;
   }
+  *theta = th;
+}
+
+// nu steps from x_0 = 0 (x_1 = D^-1 b / theta is formed on load by the first
+// product), leaving x_nu in L.x; with `resid`, then r = b - A x_nu in L.r.
+static int presmooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st,
+                     const int* skip, bool resid) {
+  double theta;
+  std::vector<ChebStep> cs;
+  cheb_steps(L.lam_max, alpha, nu, &theta, cs);
+  float* X[3] = {L.x, L.d, L.d2};
+  auto slot = [&](int j) { return X[((j - nu) % 3 + 3) % 3]; };   // x_nu in L.x
+  if (nu == 1) {
+    VOX8_CHECK(resid, "smoothing from zero needs two steps or a residual");
+    // x_1 = D^-1 b / theta and r = b - A x_1 (the first step of the residual form)
+    Epilogue<float> ep = step_epilogue(L, EPI_CHEB0, theta, {0.0, 0.0}, skip);
+    ep.r = L.r;
+    ep.xs = L.x;
+    return level_apply(L, L.b, nullptr, ncols, st, ep);
+  }
+  for (int k = 1; k < nu; ++k) {
+    Epilogue<float> ep = step_epilogue(L, k == 1 ? EPI_X0 : EPI_X, theta, cs[k - 1], skip);
+    ep.prev = k == 2 ? 2 : 1;                       // x_1 is never stored: c0 D^-1 b
+    ep.xprev = k >= 3 ? slot(k - 1) : nullptr;
+    ep.xs = slot(k + 1);
+    VOX8_TRY(level_apply(L, k == 1 ? L.b : slot(k), nullptr, ncols, st, ep));
+  }
+  if (!resid) return OK;
+  Epilogue<float> ep = step_epilogue(L, EPI_RESID, theta, {0.0, 0.0}, skip);
+  ep.r = L.r;                                        // no direction: dnew stays null
+  return level_apply(L, L.x, nullptr, ncols, st, ep);
+}
+
+// nu steps from x_0 = L.x.  The result goes to out64 (FP64, finest level) or
+// stays in one of the level's buffers, returned in *xout.
+static int postsmooth(const MGLevel& L, int nu, double alpha, int ncols, hipStream_t st,
+                      const int* skip, double* out64, const float** xout) {
+  VOX8_CHECK(!out64 || !L.S32, "FP64 smoothing output needs the structured stencil");
+  double theta;
+  std::vector<ChebStep> cs;
+  cheb_steps(L.lam_max, alpha, nu, &theta, cs);
+  float* X[3] = {L.x, L.d, L.d2};
+  for (int k = 0; k < nu; ++k) {
+    Epilogue<float> ep = step_epilogue(L, EPI_X, theta, k == 0 ? ChebStep{0.0, 1.0 / theta} : cs[k - 1],
+                                       skip);
+    ep.prev = k == 0 ? 0 : 1;
+    ep.xprev = k == 0 ? nullptr : X[(k - 1) % 3];
+    ep.xs = X[(k + 1) % 3];
+    if (k == nu - 1 && out64) {
+      ep.xs = nullptr;
+      ep.out64 = out64;
+    }
+    VOX8_TRY(level_apply(L, X[k % 3], nullptr, ncols, st, ep));
+  }
+  if (xout) *xout = out64 ? nullptr : X[nu % 3];
   return OK;
 }
 
-// Cycle on level l: L.x = MG_l(L.b).  Pre-smoothing from zero, gamma
+// Cycle on level l: x = MG_l(L.b).  Pre-smoothing from zero, gamma
 // coarse-grid corrections (each: restrict the residual, cycle on l+1, add the
-// prolonged correction, recompute the residual and the first Chebyshev
-// direction), post-smoothing.  Levels below w_min_dofs take one correction.  The coarsest level is the dense inverse, or
-// 8 nu smoothing steps from zero when it is too large for one.  With the same
-// polynomial before and after and symmetric transfers the operator is
-// symmetric, as PCG requires.
-// out64 (finest level only): where the cycle's result goes as FP64; the
-// post-smoothing sweep's last step writes it (L.x is then not updated).
-// pending (coarser levels): as in smooth(); stays null when the level ends
-// without such a step (dense coarsest level).
+// prolonged correction, and, before another correction, recompute the
+// residual), post-smoothing.  Levels below w_min_dofs take one correction.
+// The coarsest level is the dense inverse, or 8 nu smoothing steps from zero
+// when it is too large for one.  With the same polynomial before and after
+// and symmetric transfers the operator is symmetric, as PCG requires.
+// out64 (finest level only): where the cycle's result goes as FP64.
+// *xout: the level buffer holding the result otherwise.
 static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* skip,
-                    double* out64 = nullptr, const float** pending = nullptr) {
+                    double* out64, const float** xout) {
   const int nl = (int)mg.lv.size();
   MGLevel& L = mg.lv[l];
   if (l == nl - 1 && mg.n_coarse) {
@@ -903,13 +922,17 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     hipLaunchKernelGGL(k_mg_coarse_apply<4>, dim3(rb, (ncols + 3) / 4), dim3(256), 0, st,
                        mg.n_coarse, ncols, (const float*)mg.Ainv, (const float*)L.b, L.x, skip);
     VOX8_HIP(hipGetLastError());
+    *xout = L.x;
     return OK;
   }
-  const double theta = 0.5 * (L.lam_max + L.lam_max / mg.alpha);
-  if (l == nl - 1) return smooth(L, 8 * mg.nu, mg.alpha, ncols, st, true, skip);   // no coarse grid below
+  if (l == nl - 1) {                                 // no coarse grid below
+    VOX8_TRY(presmooth(L, 8 * mg.nu, mg.alpha, ncols, st, skip, false));
+    *xout = L.x;
+    return OK;
+  }
   const int nu = (l < 8 && mg.nu_level[l] > 0) ? mg.nu_level[l]
                  : (l >= mg.nu_coarse_level && mg.nu_coarse > 0) ? mg.nu_coarse : mg.nu;
-  VOX8_TRY(smooth(L, nu, mg.alpha, ncols, st, true, skip));
+  VOX8_TRY(presmooth(L, nu, mg.alpha, ncols, st, skip, true));
   MGLevel& Cn = mg.lv[l + 1];
   const bool exact_below = (l + 1 == nl - 1) && mg.n_coarse;
   int g = (exact_below || L.g.n_dofs < mg.w_min_dofs || l < mg.v_top) ? 1 : mg.gamma;
@@ -918,32 +941,21 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
     hipLaunchKernelGGL(k_mg_restrict, dim3((unsigned)((Cn.g.n_nodes + 255) / 256), ncols), dim3(256), 0, st,
                        L.g, Cn.g, (const float*)L.r, (const uint8_t*)Cn.fixed, Cn.b, skip);
     VOX8_HIP(hipGetLastError());
-    // the coarse level's last post-smoothing step is folded into the
-    // prolongation below it, except into the finest level: there the
-    // prolongation is a wide gather-bound pass, and reading the direction
-    // beside the iterate costs more than the separate add on the coarse grid
-    const float* dpend = nullptr;
-    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip, nullptr, l > 0 ? &dpend : nullptr));
-    const dim3 pgrid((unsigned)((L.g.n_nodes + 255) / 256), ncols);
-    if (dpend)
-      hipLaunchKernelGGL(k_mg_prolong_add<true>, pgrid, dim3(256), 0, st, Cn.g, L.g, (const float*)Cn.x,
-                         dpend, (const uint8_t*)L.fixed, L.x, skip);
-    else
-      hipLaunchKernelGGL(k_mg_prolong_add<false>, pgrid, dim3(256), 0, st, Cn.g, L.g, (const float*)Cn.x,
-                         (const float*)nullptr, (const uint8_t*)L.fixed, L.x, skip);
+    const float* xc = nullptr;
+    VOX8_TRY(mg_cycle(mg, l + 1, ncols, st, skip, nullptr, &xc));
+    hipLaunchKernelGGL(k_mg_prolong_add, dim3((unsigned)((L.g.n_nodes + 255) / 256), ncols), dim3(256), 0,
+                       st, Cn.g, L.g, xc, (const uint8_t*)L.fixed, L.x, skip);
     VOX8_HIP(hipGetLastError());
-    Epilogue<float> ep;
-    ep.mode = EPI_RESID;
-    ep.b = L.b;
-    ep.r = L.r;
-    ep.dnew = L.d;
-    ep.dinv = L.dinv;
-    ep.c2 = (float)(1.0 / theta);
-    ep.ld = L.g.n_dofs;
-    ep.skip = skip;
-    VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
+    if (j + 1 < g) {                                 // the next correction's residual
+      double theta;
+      std::vector<ChebStep> cs;
+      cheb_steps(L.lam_max, mg.alpha, 1, &theta, cs);
+      Epilogue<float> ep = step_epilogue(L, EPI_RESID, theta, {0.0, 0.0}, skip);
+      ep.r = L.r;
+      VOX8_TRY(level_apply(L, L.x, nullptr, ncols, st, ep));
+    }
   }
-  return smooth(L, nu, mg.alpha, ncols, st, false, skip, out64, pending);
+  return postsmooth(L, nu, mg.alpha, ncols, st, skip, out64, xout);
 }
 
 static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,
@@ -955,10 +967,11 @@ static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncol
   // With a coarser level below, the finest level's cycle ends with a
   // post-smoothing sweep, whose last step writes z directly.
   const bool fused_out = mg.lv.size() > 1;
-  VOX8_TRY(mg_cycle(mg, 0, ncols, st, skip, fused_out ? z : nullptr));
+  const float* xo = nullptr;
+  VOX8_TRY(mg_cycle(mg, 0, ncols, st, skip, fused_out ? z : nullptr, &xo));
   if (!fused_out) {
     hipLaunchKernelGGL(k_f32_to_f64, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs,
-                       (const float*)F.x, z, skip);
+                       xo, z, skip);
     VOX8_HIP(hipGetLastError());
   }
   return OK;

@@ -236,7 +236,7 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
   // which returns 0.
   const GalRsrc rs = gal_rsrc(S, 4u * 128u * (uint32_t)nn);
   const GalRsrc rx = gal_rsrc(x, 4u * (uint32_t)(ncols * ld));
-  const GalRsrc rdi = gal_rsrc(ep.dinv, MODE == EPI_CHEB0 ? 12u * (uint32_t)nn : 0u);
+  const GalRsrc rdi = gal_rsrc(ep.dinv, (MODE == EPI_CHEB0 || MODE == EPI_X0) ? 12u * (uint32_t)nn : 0u);
   uint32_t xoff[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) xoff[k] = 4u * (uint32_t)((c0 + k < ncols ? c0 + k : ncols - 1) * ld);
@@ -263,7 +263,7 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
       u = b[5]; b[5] = b[7]; b[7] = u;
     }
     float s0 = 1.0f, s1 = 1.0f, s2 = 1.0f;
-    if (MODE == EPI_CHEB0) {                           // x holds b; the input is c0 D^-1 b
+    if (MODE == EPI_CHEB0 || MODE == EPI_X0) {         // x holds b; the input is c0 D^-1 b
       s0 = c0f * gal_ld(rdi, vx, 0u);
       s1 = c0f * gal_ld(rdi, vx + 4u, 0u);
       s2 = c0f * gal_ld(rdi, vx + 8u, 0u);
@@ -308,20 +308,31 @@ k_gal_apply(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
     } else if (MODE == EPI_RESID) {
       const float r = fx ? 0.0f : ep.b[o + c] - val;
       ep.r[o + c] = r;
-      ep.dnew[o + c] = ep.c2 * ep.dinv[3 * n + c] * r;
+      if (ep.dnew) ep.dnew[o + c] = ep.c2 * ep.dinv[3 * n + c] * r;
     } else if (MODE == EPI_CHEB) {
       const float d = x[o + c];
       ep.xs[o + c] += d;
       const float r = ep.r[o + c] - val;
       ep.r[o + c] = r;
       ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+    } else if (MODE == EPI_X0) {                           // x holds b; x_1 = c0 D^-1 b
+      const float bv = x[o + c];
+      const float x1 = c0f * ep.dinv[3 * n + c] * bv;
+      const float r = fx ? 0.0f : bv - val;
+      ep.xs[o + c] = fx ? 0.0f : x1 + ep.c1 * x1 + ep.c2 * ep.dinv[3 * n + c] * r;
+    } else if (MODE == EPI_X) {                            // x holds x_k
+      const float xk = x[o + c], bv = ep.b[o + c], di = ep.dinv[3 * n + c];
+      const float xp = ep.prev == 1 ? ep.xprev[o + c] : (ep.prev == 2 ? c0f * di * bv : 0.0f);
+      const float dx = ep.prev == 0 ? 0.0f : xk - xp;
+      const float r = fx ? 0.0f : bv - val;
+      ep.xs[o + c] = fx ? 0.0f : xk + ep.c1 * dx + ep.c2 * di * r;
     } else {                                               // EPI_CHEB0 (x holds b)
       const float bv = x[o + c];
       const float d = c0f * ep.dinv[3 * n + c] * bv;
       ep.xs[o + c] = d;
       const float r = fx ? 0.0f : bv - val;
       ep.r[o + c] = r;
-      ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+      if (ep.dnew) ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
     }
   }
 }
@@ -371,7 +382,7 @@ k_gal_apply_lane(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
   const float c0f = ep.c0;
   const GalRsrc rs = gal_rsrc(S, 4u * 126u * nn);
   const GalRsrc rx = gal_rsrc(x, 4u * (uint32_t)(ncols * ld));
-  const GalRsrc rdi = gal_rsrc(ep.dinv, MODE == EPI_CHEB0 ? 12u * nn : 0u);
+  const GalRsrc rdi = gal_rsrc(ep.dinv, (MODE == EPI_CHEB0 || MODE == EPI_X0) ? 12u * nn : 0u);
   uint32_t xoff[NC];
 #pragma unroll
   for (int k = 0; k < NC; ++k) xoff[k] = 4u * (uint32_t)((c0 + k < ncols ? c0 + k : ncols - 1) * ld);
@@ -412,7 +423,7 @@ k_gal_apply_lane(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
     float s[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      if (MODE == EPI_CHEB0) {                                // x holds b; the input is c0 D^-1 b
+      if (MODE == EPI_CHEB0 || MODE == EPI_X0) {              // x holds b; the input is c0 D^-1 b
         const float dc = gal_ld(rdi, vc + 4u * c, 0u), de = gal_ld(rdi, ve + 4u * c, 0u);
         s[1][c] = c0f * dc;
         s[0][c] = c0f * lane_prev_or(de, dc);
@@ -458,20 +469,31 @@ k_gal_apply_lane(Grid g, int ncols, int wg_per_xcd, const float* __restrict__ S,
       } else if (MODE == EPI_RESID) {
         const float r = fx ? 0.0f : ep.b[o + c] - val;
         ep.r[o + c] = r;
-        ep.dnew[o + c] = ep.c2 * ep.dinv[3 * n + c] * r;
+        if (ep.dnew) ep.dnew[o + c] = ep.c2 * ep.dinv[3 * n + c] * r;
       } else if (MODE == EPI_CHEB) {
         const float d = x[o + c];
         ep.xs[o + c] += d;
         const float r = ep.r[o + c] - val;
         ep.r[o + c] = r;
         ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
-      } else {                                             // EPI_CHEB0 (x holds b)
+      } else if (MODE == EPI_X0) {                           // x holds b; x_1 = c0 D^-1 b
+        const float bv = x[o + c];
+        const float x1 = c0f * ep.dinv[3 * n + c] * bv;
+        const float r = fx ? 0.0f : bv - val;
+        ep.xs[o + c] = fx ? 0.0f : x1 + ep.c1 * x1 + ep.c2 * ep.dinv[3 * n + c] * r;
+      } else if (MODE == EPI_X) {                            // x holds x_k
+        const float xk = x[o + c], bv = ep.b[o + c], di = ep.dinv[3 * n + c];
+        const float xp = ep.prev == 1 ? ep.xprev[o + c] : (ep.prev == 2 ? c0f * di * bv : 0.0f);
+        const float dx = ep.prev == 0 ? 0.0f : xk - xp;
+        const float r = fx ? 0.0f : bv - val;
+        ep.xs[o + c] = fx ? 0.0f : xk + ep.c1 * dx + ep.c2 * di * r;
+      } else {                                               // EPI_CHEB0 (x holds b)
         const float bv = x[o + c];
         const float d = c0f * ep.dinv[3 * n + c] * bv;
         ep.xs[o + c] = d;
         const float r = fx ? 0.0f : bv - val;
         ep.r[o + c] = r;
-        ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
+        if (ep.dnew) ep.dnew[o + c] = ep.c1 * d + ep.c2 * ep.dinv[3 * n + c] * r;
       }
     }
   }
@@ -525,6 +547,8 @@ static void gal_apply_nc(const Grid& g, const float* S32, bool by_entry, const u
     case EPI_RESID: hipLaunchKernelGGL((KERN<EPI_RESID, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break; \
     case EPI_CHEB: hipLaunchKernelGGL((KERN<EPI_CHEB, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;   \
     case EPI_CHEB0: hipLaunchKernelGGL((KERN<EPI_CHEB0, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break; \
+    case EPI_X0: hipLaunchKernelGGL((KERN<EPI_X0, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;       \
+    case EPI_X: hipLaunchKernelGGL((KERN<EPI_X, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;         \
     default: hipLaunchKernelGGL((KERN<EPI_PLAIN, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;        \
   }
   static const int unr = getenv("VOX8_GAL_UNR") ? atoi(getenv("VOX8_GAL_UNR")) : 1;

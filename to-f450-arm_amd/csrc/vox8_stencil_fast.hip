@@ -288,7 +288,9 @@ __device__ __forceinline__ void element_M(const FastCoef<T>& C, T fm, const T mo
 // EPI_CHEB one Chebyshev smoothing step with x holding the direction d:
 //   xs += d, r -= A d, d' = c1 d + c2 D^-1 r   (all node-local once A d is known);
 // EPI_CHEB0 the same step from xs = 0, r = b with x holding b: d = c0 D^-1 b
-//   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r.
+//   is formed while staging, then xs = d, r = b - A d, d' = c1 d + c2 D^-1 r;
+// EPI_X0 / EPI_X (FP32) a step of the three-term iterate form (EpiMode): only
+//   x_{k+1} is stored (into xs, or as FP64 into out64).
 template <typename T, int PAT, bool DO_K, bool DO_M, bool FUSE, int MODE, int TY>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !DO_M ? 6 : 1)))
 k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T> C,
@@ -321,11 +323,16 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   x += col * ldx;
   if (DO_K && MODE == EPI_PLAIN) yK += col * ldy;
   if (DO_M) yM += col * ldy;
+  constexpr bool XF = MODE == EPI_X0 || MODE == EPI_X;          // three-term iterate form
   if (MODE != EPI_PLAIN) {
-    ep.r += col * ep.ld;
-    ep.dnew += col * ep.ld;
-    if (MODE == EPI_RESID) ep.b += col * ep.ld;
-    if (MODE == EPI_CHEB || MODE == EPI_CHEB0) ep.xs += col * ep.ld;
+    if (!XF) {
+      ep.r += col * ep.ld;
+      if (ep.dnew) ep.dnew += col * ep.ld;
+    }
+    if (MODE == EPI_RESID || MODE == EPI_X) ep.b += col * ep.ld;
+    if (MODE != EPI_RESID && ep.xs) ep.xs += col * ep.ld;
+    if (MODE == EPI_X && ep.prev == 1) ep.xprev += col * ep.ld;
+    if (MODE == EPI_X && ep.out64) ep.out64 += col * ep.ld;
   }
   const int tx_t = tile % tiles_x, rest = tile / tiles_x;
   const int ty_t = rest % tiles_y, zc_t = rest / tiles_y;
@@ -358,11 +365,14 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const Rsrc rfk = make_rsrc(fK, DO_K ? ebytes : 0u);
   const Rsrc rfm = make_rsrc(fM, DO_M ? ebytes : 0u);
   const Rsrc rdi = make_rsrc(ep.dinv, MODE != EPI_PLAIN ? vbytes : 0u);
-  const Rsrc rb = make_rsrc(MODE == EPI_RESID ? ep.b : (MODE == EPI_CHEB ? ep.r : x),
+  const Rsrc rb = make_rsrc((MODE == EPI_RESID || MODE == EPI_X) ? ep.b : (MODE == EPI_CHEB ? ep.r : x),
                             MODE == EPI_PLAIN ? 0u : vbytes);        // epilogue input a
-  const Rsrc rxs = make_rsrc(ep.xs, (MODE == EPI_CHEB || MODE == EPI_CHEB0) ? vbytes : 0u);
-  const Rsrc ry1 = make_rsrc(MODE == EPI_PLAIN ? yK : ep.r, DO_K ? vbytes : 0u);
-  const Rsrc ry2 = make_rsrc(ep.dnew, MODE != EPI_PLAIN ? vbytes : 0u);
+  const Rsrc rxs = make_rsrc(ep.xs, (MODE == EPI_CHEB || MODE == EPI_CHEB0 || (XF && !ep.out64))
+                                        ? vbytes : 0u);
+  const Rsrc rxp = make_rsrc(ep.xprev, (MODE == EPI_X && ep.prev == 1) ? vbytes : 0u);
+  const Rsrc r64 = make_rsrc(ep.out64, (MODE == EPI_X && ep.out64) ? 2u * vbytes : 0u);
+  const Rsrc ry1 = make_rsrc(MODE == EPI_PLAIN ? yK : ep.r, (DO_K && !XF) ? vbytes : 0u);
+  const Rsrc ry2 = make_rsrc(ep.dnew, (MODE != EPI_PLAIN && !XF && ep.dnew) ? vbytes : 0u);
   const Rsrc rym = make_rsrc(yM, DO_M ? vbytes : 0u);
 
   // plane staging: thread loads nodes q = t + i NT of the (PX x PY) plane;
@@ -379,7 +389,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     qoff[i] = qin[i] ? (uint32_t)nxg + (uint32_t)g.NX * (uint32_t)nyg : 0u;
   }
   T pre[NLD][3];
-  T prd[NLD][3];                                               // EPI_CHEB0: D^-1 of the plane
+  T prd[NLD][3];                                               // CHEB0, X0: D^-1 of the plane
   uint32_t prf[NLD];
   auto fetch = [&](int iz) {
     const bool zin = iz >= 0 && iz < g.NZ;
@@ -390,7 +400,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
       const uint32_t o3 = in ? 3u * (uint32_t)sizeof(T) * n : OOB, o1 = in ? n : OOB;
       prf[i] = ld_u8(rfix, o1);
       ld3(rx, o3, pre[i]);
-      if (MODE == EPI_CHEB0) ld3(rdi, o3, prd[i]);
+      if (MODE == EPI_CHEB0 || MODE == EPI_X0) ld3(rdi, o3, prd[i]);
     }
   };
   auto store = [&](int slot) {
@@ -401,7 +411,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           T v = pre[i][c];
-          if (MODE == EPI_CHEB0) v *= ep.c0 * prd[i][c];
+          if (MODE == EPI_CHEB0 || MODE == EPI_X0) v *= ep.c0 * prd[i][c];
           pl[slot][c][q] = ((prf[i] >> c) & 1u) ? T(0) : v;   // out of range: v = 0
         }
     }
@@ -417,9 +427,9 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   // used after the barrier.
   struct NodeOps {
     uint32_t fnode, off;
-    T a[3];    // RESID: b; CHEB: r; CHEB0: b (= x)
+    T a[3];    // RESID, X: b; CHEB: r; CHEB0, X0: b (= x)
     T di[3];   // D^-1
-    T xs[3];   // CHEB: iterate
+    T xs[3];   // CHEB: iterate; X: x_{k-1} (prev 1)
   };
   auto load_fac = [&](int L, T& fk, T& fm) {
     const bool ein = exy_in && L >= 0 && L < g.nz;
@@ -437,6 +447,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
       ld3(rdi, o.off, o.di);
     }
     if (MODE == EPI_CHEB) ld3(rxs, o.off, o.xs);
+    if (MODE == EPI_X) ld3(rxp, o.off, o.xs);
   };
   // Hadamard coefficients of the x-y face of the element at (tx, ty) in a
   // node plane: per component [s+, s-, d+, d-] with s/d the x sum/difference
@@ -553,6 +564,14 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
           const T r = cur.a[c] - val;
           o1[c] = r;
           o2[c] = ep.c1 * xin[c] + ep.c2 * cur.di[c] * r;
+        } else if (MODE == EPI_X0) {                            // xin = x_1 = c0 D^-1 b
+          const T r = fx ? T(0) : cur.a[c] - val;
+          o3[c] = fx ? T(0) : xin[c] + ep.c1 * xin[c] + ep.c2 * cur.di[c] * r;
+        } else if (MODE == EPI_X) {                             // xin = x_k
+          const T xp = ep.prev == 2 ? ep.c0 * cur.di[c] * cur.a[c] : cur.xs[c];
+          const T dx = ep.prev == 0 ? T(0) : xin[c] - xp;
+          const T r = fx ? T(0) : cur.a[c] - val;
+          o3[c] = fx ? T(0) : xin[c] + ep.c1 * dx + ep.c2 * cur.di[c] * r;
         } else {                                                // EPI_CHEB0
           o3[c] = xin[c];
           const T r = fx ? T(0) : cur.a[c] - val;
@@ -565,6 +584,13 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     if (DO_K) {
       if (MODE == EPI_PLAIN) {
         st3(ry1, cur.off, vK);
+      } else if (XF) {
+        if (MODE == EPI_X && ep.out64) {
+          const double w[3] = {(double)o3[0], (double)o3[1], (double)o3[2]};
+          st3(r64, cur.off == OOB ? OOB : 2u * cur.off, w);
+        } else {
+          st3(rxs, cur.off, o3);
+        }
       } else {
         st3(ry1, cur.off, o1);
         st3(ry2, cur.off, o2);
@@ -889,6 +915,11 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     set_error("stencil epilogue modes apply to K alone");
     return ERR_ARG;
   }
+  if ((ep.mode == EPI_X0 || ep.mode == EPI_X) &&
+      (sizeof(T) != 4 || (ep.out64 && 3 * g.n_nodes * (int64_t)sizeof(double) >= ((int64_t)1 << 31)))) {
+    set_error("three-term Chebyshev epilogue: FP32 only, FP64 output below 2^31 bytes per column");
+    return ERR_ARG;
+  }
   // one launch at a given shape (the kernel instance follows the pattern,
   // operator, fusion, epilogue and tile rows)
   auto launch = [&](int ty, int zc) {
@@ -912,6 +943,12 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     if (ep.mode == EPI_RESID) VOX8_LAUNCH(PAT, true, false, false, EPI_RESID);          \
     else if (ep.mode == EPI_CHEB) VOX8_LAUNCH(PAT, true, false, false, EPI_CHEB);       \
     else if (ep.mode == EPI_CHEB0) VOX8_LAUNCH(PAT, true, false, false, EPI_CHEB0);     \
+    else if (ep.mode == EPI_X0 || ep.mode == EPI_X) {                                   \
+      if constexpr (sizeof(T) == 4) {                                                   \
+        if (ep.mode == EPI_X0) VOX8_LAUNCH(PAT, true, false, false, EPI_X0);            \
+        else VOX8_LAUNCH(PAT, true, false, false, EPI_X);                               \
+      }                                                                                 \
+    }                                                                                   \
     else if (fz) VOX8_LAUNCH(PAT, true, false, true, EPI_PLAIN);                         \
     else VOX8_LAUNCH(PAT, true, false, false, EPI_PLAIN);                               \
   } else if (op == OP_M) VOX8_LAUNCH(PAT, false, true, false, EPI_PLAIN);                \
