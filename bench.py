@@ -10,7 +10,8 @@ in HBM before the timed region.
 
 Prints one JSON line.  Extra objects:
   roofline      the dominant kernel (FP32 Chebyshev smoothing step of the
-                multigrid preconditioner on the fine grid, 6 columns), timed
+                multigrid preconditioner on the fine grid, three-term form on
+                the iterates, 6 columns), timed
                 with HIP events on the solver stream; algorithmic bytes per
                 launch in "alg_bytes_per_launch"; "traffic" from the committed
                 rocprofv3 FETCH_SIZE/WRITE_SIZE summary when present
@@ -48,7 +49,7 @@ for _p in (ROOT, ROOT / "to-f450-arm_amd"):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0
-DOMINANT = "cheb_fp32"
+DOMINANT = "cheb_x_fp32"
 
 
 def make_optimizer(wl_name, precision, modal_maxiter=2000, galerkin=True, modal_guard="auto"):
@@ -120,15 +121,24 @@ def kernel_rooflines(fem, geom, rho, nrhs=6, reps=30):
     if fem.precision == "mixed":
         rz = torch.zeros(nrhs, n, dtype=torch.float64, device=fem.device)
         fem._call("vox8_precond_apply", _lib.ptr(rz), _lib.ptr(rz), nrhs)
-    # FP32 Chebyshev smoothing step (multigrid fine level), nrhs columns
+    # FP32 Chebyshev smoothing step of the multigrid's fine level (three-term
+    # form on the iterates), nrhs columns
     d = torch.randn(nrhs, n, dtype=torch.float32, device=fem.device)
     r = torch.randn_like(d)
-    xs = torch.zeros_like(d)
+    xs = torch.randn_like(d)
     dn = torch.empty_like(d)
+    t = _event_time(fem, lambda: fem._call("vox8_cheb_xstep", _lib.ptr(d), _lib.ptr(r), _lib.ptr(xs),
+                                           _lib.ptr(dn), 0.5, 1e-3, nrhs), reps)
+    alg = nrhs * 16 * n + 4 * n + 4 * ne + nn
+    out["cheb_x_fp32"] = dict(kernel=f"k_apply_fast<float,CUBE,K,EPI_X> (FP32 Chebyshev step, {nrhs} columns)",
+                              t=t, alg=alg, per_unit="16 B per DOF and column (read x_k, x_k-1, b; write "
+                                                     "x_k+1), + 4 B/DOF D^-1, 4 B/element factor, 1 B/node mask")
+    # the same step in the residual-direction form (vox8_cheb_step)
     t = _event_time(fem, lambda: fem._call("vox8_cheb_step", _lib.ptr(d), _lib.ptr(r), _lib.ptr(xs),
                                            _lib.ptr(dn), 0.5, 1e-3, nrhs), reps)
     alg = nrhs * 24 * n + 4 * n + 4 * ne + nn
-    out["cheb_fp32"] = dict(kernel=f"k_apply_fast<float,CUBE,K,EPI_CHEB> (FP32 Chebyshev step, {nrhs} columns)",
+    out["cheb_fp32"] = dict(kernel=f"k_apply_fast<float,CUBE,K,EPI_CHEB> (FP32 Chebyshev step, residual "
+                                   f"form, {nrhs} columns)",
                             t=t, alg=alg, per_unit="24 B per DOF and column (read d, r, xs; write r, xs, d'), "
                                                    "+ 4 B/DOF D^-1, 4 B/element factor, 1 B/node mask")
     # FP64 fused K and M products (LOBPCG), nrhs columns

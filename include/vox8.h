@@ -132,6 +132,16 @@ int vox8_precond_apply(vox8_ctx* ctx, const double* d_r, double* d_z, int nrhs);
  * and for timing the dominant kernel.  Needs the structured stencil. */
 int vox8_cheb_step(vox8_ctx* ctx, const float* d_d, float* d_r, float* d_xs, float* d_dnew,
                    double c1, double c2, int nrhs);
+/* One FP32 Chebyshev smoothing step in the three-term form on the iterates,
+ * the step the multigrid preconditioner runs on the fine grid:
+ * d_xnew = d_xk + c1 (d_xk - d_xprev) + c2 D^-1 (d_b - K d_xk) (D = diag K;
+ * d_xprev may be null: the c1 term is then dropped), zero at fixed DOFs;
+ * nrhs column-major vectors (ld = n_dofs), d_xnew aliasing no input.  No
+ * reference counterpart (the reference preconditions with the Jacobi
+ * diagonal only, src/fem.py:543-547); exposed for custom smoothers and for
+ * timing the dominant kernel.  Needs the structured stencil. */
+int vox8_cheb_xstep(vox8_ctx* ctx, const float* d_xk, const float* d_xprev, const float* d_b,
+                    float* d_xnew, double c1, double c2, int nrhs);
 
 /* Level hierarchy: element counts per level (h_dims[3*l..3*l+2]) and the
  * smoother's lambda_max(D^-1 K) estimate per level. */

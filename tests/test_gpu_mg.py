@@ -160,6 +160,36 @@ def test_cheb_step_matches_fp64_products(gpu):
     assert rel(dn.double().cpu().numpy(), dn_ref.cpu().numpy()) < 1e-5
 
 
+@pytest.mark.parametrize("with_prev", [True, False])
+def test_cheb_xstep_matches_fp64_products(gpu, with_prev):
+    """vox8_cheb_xstep (FP32, 3 columns) against FP64 K products and diagonal:
+    x' = x + c1 (x - x_prev) + c2 D^-1 (b - K x), zero at fixed DOFs (without
+    x_prev the c1 term is dropped).  FP32 bar 1e-5 relative."""
+    from vox8 import _lib
+    geom, fem, rho, _ = _setup((70, 10, 9), 5, 0.1)
+    n = geom.n_dofs
+    rng = np.random.default_rng(8)
+    free = torch.zeros(n, dtype=torch.float64, device=gpu)
+    free[torch.as_tensor(fem.free_dofs, device=gpu)] = 1.0
+    x = (torch.tensor(rng.standard_normal((3, n)), device=gpu) * free).float().contiguous()
+    xp = (torch.tensor(rng.standard_normal((3, n)), device=gpu) * free).float().contiguous()
+    b = (torch.tensor(rng.standard_normal((3, n)), device=gpu) * free * 1e6).float().contiguous()
+    xn = torch.full_like(x, 7.0)
+    c1, c2 = 0.7, 3e-7
+    fem._call("vox8_cheb_xstep", _lib.ptr(x), _lib.ptr(xp) if with_prev else None, _lib.ptr(b),
+              _lib.ptr(xn), c1, c2, 3)
+    K = fem.assemble_K(rho)
+    Kx = (K @ x.double().T).T                         # zero rows at fixed DOFs
+    diag = torch.as_tensor(K.diagonal(), device=gpu)
+    dinv = torch.where(free > 0, 1.0 / torch.where(diag != 0, diag, torch.ones_like(diag)),
+                       torch.zeros_like(diag))
+    xd = x.double()
+    mom = c1 * (xd - xp.double()) if with_prev else 0.0
+    want = (xd + mom + c2 * dinv * (b.double() - Kx)) * free
+    assert rel(xn.double().cpu().numpy(), want.cpu().numpy()) < 1e-5
+    assert torch.all(xn.double()[:, free == 0] == 0)
+
+
 def test_graph_replays_match_eager(gpu):
     """A multigrid cycle replayed from its captured HIP graph gives the same
     result, bitwise, as launching its kernels one by one: on every replay,

@@ -444,6 +444,33 @@ int vox8_cheb_step(vox8_ctx* c, const float* d_d, float* d_r, float* d_xs, float
   API_END
 }
 
+int vox8_cheb_xstep(vox8_ctx* c, const float* d_xk, const float* d_xprev, const float* d_b,
+                    float* d_xnew, double c1, double c2, int nrhs) {
+  API_BEGIN
+  VOX8_TRY(require(c, true));
+  VOX8_CHECK(d_xk && d_b && d_xnew && nrhs >= 1 && nrhs <= 64, "bad argument");
+  VOX8_CHECK(d_xnew != d_xk && d_xnew != d_xprev && d_xnew != d_b, "d_xnew must not alias an input");
+  if (c->pattern == PAT_DENSE || c->stencil_kind != 0) {
+    set_error("vox8_cheb_xstep needs the structured stencil (box elements)");
+    return ERR_STATE;
+  }
+  VOX8_TRY(ensure_dinv(c, 0));
+  Epilogue<float> ep;
+  ep.mode = EPI_X;
+  ep.b = d_b;
+  ep.xs = d_xnew;
+  ep.xprev = d_xprev;
+  ep.prev = d_xprev ? 1 : 0;
+  ep.dinv = c->dinvK32;
+  ep.c1 = (float)c1;
+  ep.c2 = (float)c2;
+  ep.ld = c->g.n_dofs;
+  return launch_apply_fast<float>(c->g, c->pattern, OP_K, c->c32, c->fK32, nullptr, c->d_fixed, d_xk,
+                                  c->g.n_dofs, nullptr, nullptr, c->g.n_dofs, nrhs, c->st, DotFuse(),
+                                  ep);
+  API_END
+}
+
 int vox8_mg_info(vox8_ctx* c, int* h_nlevels, int* h_dims, double* h_lam_max, int max_levels) {
   API_BEGIN
   VOX8_TRY(require(c, true));

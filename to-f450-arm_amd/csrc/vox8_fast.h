@@ -78,10 +78,12 @@ struct StencilShape {
 };
 
 // Autotune the FP32 launch shape for a grid and column count: times a
-// Chebyshev smoothing step of each candidate shape on d, r, xs, dnew (ncols
-// columns, leading dimension ld; r, xs, dnew are overwritten, so pass work
-// vectors) and keeps the choice for the process: every FP32 launch on such a
-// grid uses it.  Synchronises the stream; not for use during graph capture.
+// Chebyshev smoothing step (three-term form: x_k in d, x_{k-1} in r, b in
+// xs, x_{k+1} written to dnew; ncols columns, leading dimension ld; pass work
+// vectors) of each candidate shape and keeps the choice for the process:
+// every FP32 launch on such a grid uses it.  Synchronises the stream; not for
+// use during graph capture.  VOX8_FP32_SHAPE="rows/chunk" fixes the shape
+// instead of timing (profiler runs).
 int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, const float* fK,
                       const uint8_t* fixed, const float* dinv, float* d, float* r, float* xs,
                       float* dnew, int64_t ld, int ncols, hipStream_t st);

@@ -54,6 +54,7 @@ SIGNATURES = {
     "vox8_lobpcg_ritz": [_vp, _pf64, _i32, _pi32],
     "vox8_precond_apply": [_vp, _vp, _vp, _i32],
     "vox8_cheb_step": [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _i32],
+    "vox8_cheb_xstep": [_vp, _vp, _vp, _vp, _vp, _f64, _f64, _i32],
     "vox8_mg_info": [_vp, _pi32, _pi32, _pf64, _i32],
     "vox8_set_mg_options": [_vp, _i32, _f64, _i32, _i32, C.c_int64],
     "vox8_set_mg_graphs": [_vp, _i32],
