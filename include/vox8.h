@@ -285,7 +285,8 @@ int vox8_compose_density(vox8_ctx* ctx, const uint8_t* d_design, const uint8_t* 
  * d_check, and h_state[20] = {tracked mode, lam, violation, stats of sens_f
  * (4), of dlam_f (4), of d_check - d_phys_prev (4: [13] = change), of d_xnew
  * on the design (4: [15] = volume sum), penalty coefficient}.  One host
- * synchronisation. */
+ * synchronisation; the launches are replayed from a captured HIP graph
+ * (vox8_set_tail_graph). */
 typedef struct vox8_tail_args {
   const double* d_X;         /* k column-major modes (ld ldx), sorted */
   const double* h_lam;       /* k eigenvalues, ascending, >= 0 */
@@ -304,6 +305,16 @@ typedef struct vox8_tail_args {
   int k, have_prev;
 } vox8_tail_args;
 int vox8_simp_tail(vox8_ctx* ctx, const vox8_tail_args* args, double* h_state);
+/* Replay vox8_simp_tail's launches from captured HIP graphs (enable 1, the
+ * default: direct launches if the runtime refuses the capture; 2: the same,
+ * but a refused capture is an error) or launch them one by one (0).  A
+ * graph is captured per set
+ * of buffers and constants (at most four kept per context; the per-call
+ * eigenvalues, penalty coefficient and mode-block address are uploaded before
+ * each replay); the results are bitwise those of the direct launches.  No
+ * reference counterpart: the reference's loop tail is host NumPy
+ * (src/topopt.py:327-412). */
+int vox8_set_tail_graph(vox8_ctx* ctx, int enable);
 
 /* Modal effective mass of k modes (columns of d_X, leading dimension ld,
  * zero at fixed DOFs) in translation direction 0 (x), 1 (y) or 2 (z), with

@@ -99,6 +99,34 @@ def test_device_tail_matches_host_tail(gpu, case):
     assert rel(runs[True]["rho_active"], runs[False]["rho_active"]) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["active", "no_modal"])
+def test_tail_graph_replays_match_direct_launches(gpu, case):
+    """vox8_simp_tail replayed from captured HIP graphs (required: a refused
+    capture fails the run) against the same launches issued one by one:
+    bitwise equal histories and designs over the whole run (the graphs are
+    replayed with new eigenvalues, penalty coefficients and mode blocks, and
+    the ping-pong design buffers select between cached graphs)."""
+    from vox8.fem import FEM3D, Material
+    from vox8.topopt import SIMPOptimizer
+    geom, loads, opts, mat = tipbox_problem()
+    opts = dict(opts)
+    if case == "no_modal":
+        opts["min_freq_Hz"] = None
+    runs = {}
+    for mode in (0, 2):
+        fem = FEM3D(geom, Material(**mat), precision="mixed")
+        fem._call("vox8_set_tail_graph", mode)
+        runs[mode] = SIMPOptimizer(geom, fem, loads, solve_rtol=TIPBOX_SOLVE_RTOL,
+                                   modal_tol=TIPBOX_MODAL_TOL, modal_maxiter=TIPBOX_MODAL_MAXITER,
+                                   device_tail=True, **opts).run(verbose=False)
+    a, b = runs[0]["history"], runs[2]["history"]
+    assert len(a["compliance"]) >= 4
+    for k in HIST:
+        x, y = np.asarray(a[k], dtype=float), np.asarray(b[k], dtype=float)
+        assert np.array_equal(x, y, equal_nan=True), (k, x, y)
+    assert np.array_equal(runs[2]["rho_active"], runs[0]["rho_active"])
+
+
 def test_auto_guard_columns_keep_the_history(gpu):
     """The "auto" guard policy (two guard columns or none, whichever has cost
     less per solve; here also a variant that tries the other setting every

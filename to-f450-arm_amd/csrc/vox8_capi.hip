@@ -76,6 +76,18 @@ struct vox8_ctx {
   // OC / sensitivities scratch
   double* d_oc = nullptr;
   double* d_tail = nullptr;         // TAIL_STATE doubles
+  // vox8_simp_tail: the launches captured as HIP graphs (keyed by every value
+  // the launches take by value; the per-call values go through d_tail_par)
+  TailPar* d_tail_par = nullptr;
+  TailPar* h_tail_par = nullptr;    // pinned staging of it
+  int tail_graphs = 1;              // 0 direct launches, 1 graphs, 2 graphs or an error
+  struct TailGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+  };
+  std::vector<TailGraph> tail_cache;
+  uint64_t tail_clock = 0;
   int lobpcg_guard = -1;            // guard columns of vox8_lobpcg (-1: environment or 0)
   double *d_kphi = nullptr, *d_mphi = nullptr, *d_ce = nullptr, *d_sums = nullptr,
          *d_wts = nullptr;
@@ -348,7 +360,8 @@ int vox8_create(vox8_ctx** out, int nx, int ny, int nz, void* hip_stream) {
       (s = dalloc(c, &c->d_mphi, ne)) || (s = dalloc(c, &c->d_ce, ne)) ||
       (s = dalloc(c, &c->d_sums, 32)) || (s = dalloc(c, &c->d_wts, 32)) ||
       (s = dalloc(c, &c->d_den, ne)) || (s = dalloc(c, &c->d_fmask, ne)) ||
-      (s = dalloc(c, &c->d_changed, 1)) || (s = halloc(c, &c->h_changed, 1)))
+      (s = dalloc(c, &c->d_changed, 1)) || (s = halloc(c, &c->h_changed, 1)) ||
+      (s = dalloc(c, &c->d_tail_par, 1)) || (s = halloc(c, &c->h_tail_par, 1)))
     return fail(s);
   // stream-ordered (a legacy-stream memset conflicts with graph captures in
   // progress on other threads)
@@ -367,6 +380,8 @@ int vox8_destroy(vox8_ctx* c) {
   HipSerial lock(hip_serial());
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st); else (void)hipDeviceSynchronize();
+  for (auto& tg : c->tail_cache) (void)hipGraphExecDestroy(tg.exec);
+  c->tail_cache.clear();
   for (void* p : c->dev_allocs) (void)hipFree(p);
   for (void* p : c->host_allocs) (void)hipHostFree(p);
   delete c;
@@ -909,6 +924,112 @@ int vox8_axpby(vox8_ctx* c, int64_t n, const uint8_t* d_mask, double a, const do
   API_END
 }
 
+namespace {
+
+// The launches of the loop tail (src/topopt.py:327-412): mode tracking by
+// MAC, the gated eigenvalue-sensitivity branch and frequency penalty, the OC
+// bisection, the density filter and the change / volume statistics.  The
+// per-call values (eigenvalues, mu, the mode block's address) are read from
+// c->d_tail_par, so the sequence is the same for every call with the same
+// buffers and constants.
+int tail_launches(vox8_ctx* c, const vox8_tail_args* a, hipStream_t st) {
+  const int64_t ne = c->g.n_elem, nd = c->g.n_dofs;
+  double* state = c->d_tail;
+  const TailPar* par = c->d_tail_par;
+  TailSelect s{};
+  s.k = a->k;
+  s.have_prev = a->have_prev;
+  s.w_target = a->w_target;
+  if (a->k > 0) {
+    if (a->have_prev)
+      VOX8_TRY(launch_block_dots(nd, a->k, a->d_X, a->ldx, a->d_vprev, c->red_partials,
+                                 c->red_counters, c->red_out, st, &par->X));
+    VOX8_TRY(launch_tail_select(s, par, c->red_out, state, st));
+    VOX8_TRY(launch_gather_col(nd, par, a->ldx, state, a->d_vprev, st));
+    // eigenvalue sensitivity branch, gated on the violation (state[2])
+    const double* gate = state + 2;
+    VOX8_TRY(launch_quadforms(c->g, c->fK, c->fM, c->d_active, c->dKe, c->dMe, nullptr, nd, 0,
+                              c->d_wts, a->d_vprev, nullptr, c->d_kphi, c->d_mphi, c->red_partials,
+                              c->red_counters, c->d_sums, c->d_tip, c->n_tip, c->tip_m, st,
+                              gate, c->pattern != PAT_DENSE ? &c->c64 : nullptr));
+    VOX8_TRY(launch_sens_finish(ne, c->d_active, a->d_rho, a->p, a->e_min, a->mthr, a->mq, c->d_ce,
+                                nullptr, c->d_kphi, c->d_mphi, 0.0, c->d_sums, a->d_dlam, st,
+                                state + 1, gate));
+    VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask,
+                           c->d_den, a->d_dlam, a->d_dlam_f, FILTER_FWD, st, gate));
+    VOX8_TRY(launch_stats(ne, a->d_design, a->d_sens_f, nullptr, c->red_partials, c->red_counters,
+                          state + 3, st, gate));
+    VOX8_TRY(launch_stats(ne, a->d_design, a->d_dlam_f, nullptr, c->red_partials, c->red_counters,
+                          state + 7, st, gate));
+    VOX8_TRY(launch_tail_penalty(ne, par, state, a->d_dlam_f, a->d_sens_f, st));
+  } else {
+    VOX8_TRY(launch_tail_select(s, par, nullptr, state, st));
+  }
+  VOX8_TRY(launch_oc(ne, a->d_design, a->d_x, a->d_sens_f, a->V_e, a->target_volume, a->move,
+                     c->d_oc, c->red_partials, c->red_counters, a->d_xnew, 100, st));
+  VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask, c->d_den,
+                         a->d_xnew, a->d_check, FILTER_FWD, st));
+  VOX8_TRY(launch_stats(ne, nullptr, a->d_check, a->d_phys_prev, c->red_partials, c->red_counters,
+                        state + 11, st));
+  VOX8_TRY(launch_stats(ne, a->d_design, a->d_xnew, nullptr, c->red_partials, c->red_counters,
+                        state + 15, st));
+  return OK;
+}
+
+// Every value tail_launches passes to a kernel by value, except those read
+// from d_tail_par: two calls with equal keys record identical launches.
+std::vector<unsigned char> tail_key(const vox8_ctx* c, const vox8_tail_args* a) {
+  std::vector<unsigned char> k;
+  auto put = [&](const auto& v) {
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+    k.insert(k.end(), p, p + sizeof(v));
+  };
+  put(a->k); put(a->have_prev); put(a->w_target); put(a->ldx);
+  put(a->d_X != nullptr); put(a->d_vprev); put(a->d_rho); put(a->d_dlam); put(a->d_dlam_f);
+  put(a->d_sens_f); put(a->d_design); put(a->d_x); put(a->d_xnew); put(a->d_check);
+  put(a->d_phys_prev); put(a->p); put(a->e_min); put(a->mthr); put(a->mq); put(a->V_e);
+  put(a->target_volume); put(a->move);
+  put(c->g); put(c->fK); put(c->fM); put(c->d_active); put(c->dKe); put(c->dMe); put(c->d_wts);
+  put(c->d_kphi); put(c->d_mphi); put(c->d_ce); put(c->d_sums); put(c->d_tip); put(c->n_tip);
+  put(c->tip_m); put(c->pattern); put(c->c64); put(c->ntap); put(c->d_toff); put(c->d_tw);
+  put(c->d_fmask); put(c->d_den); put(c->d_oc); put(c->d_tail); put(c->d_tail_par);
+  put(c->red_partials); put(c->red_counters); put(c->red_out);
+  return k;
+}
+
+// Capture tail_launches on the context's stream (or, when that is the null
+// stream, which cannot be captured, on the multigrid's private capture
+// stream) under the process-wide HIP lock, and instantiate.  Returns false,
+// with no error pending, when the runtime rejects the capture.
+bool tail_capture(vox8_ctx* c, const vox8_tail_args* a, hipGraphExec_t* exec) {
+  HipSerial lock(hip_serial());
+  *exec = nullptr;
+  if (!c->st && !c->mg.cap_st &&
+      hipStreamCreateWithFlags(&c->mg.cap_st, hipStreamNonBlocking) != hipSuccess) {
+    c->mg.cap_st = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  hipStream_t cs = c->st ? c->st : c->mg.cap_st;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  const int rc = tail_launches(c, a, cs);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(cs, &graph);
+  bool ok = rc == OK && ec == hipSuccess && graph != nullptr;
+  if (ok) ok = hipGraphInstantiateWithFlags(exec, graph, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    *exec = nullptr;
+    (void)hipGetLastError();
+  }
+  return ok;
+}
+
+}  // namespace
+
 int vox8_simp_tail(vox8_ctx* c, const vox8_tail_args* a, double* h_state) {
   API_BEGIN
   VOX8_TRY(require(c, true));
@@ -918,52 +1039,68 @@ int vox8_simp_tail(vox8_ctx* c, const vox8_tail_args* a, double* h_state) {
              "tail: null element field");
   VOX8_CHECK(a->V_e > 0.0, "V_e must be positive");
   if (!c->have_filter) { set_error("filter not set up (vox8_filter_setup)"); return ERR_STATE; }
-  const int64_t ne = c->g.n_elem, nd = c->g.n_dofs;
-  double* state = c->d_tail;
-  TailSelect s{};
-  s.k = a->k;
-  s.have_prev = a->have_prev;
-  s.w_target = a->w_target;
   if (a->k > 0) {
     VOX8_CHECK(a->d_X && a->h_lam && a->d_vprev && a->d_rho && a->d_dlam && a->d_dlam_f,
                "tail: null modal argument");
-    VOX8_CHECK(a->ldx >= nd, "tail: ldx < n_dofs");
+    VOX8_CHECK(a->ldx >= c->g.n_dofs, "tail: ldx < n_dofs");
     VOX8_CHECK(a->w_target > 0.0, "tail: w_target must be positive");
-    for (int j = 0; j < a->k; ++j) s.lam[j] = a->h_lam[j];
-    if (a->have_prev)
-      VOX8_TRY(launch_block_dots(nd, a->k, a->d_X, a->ldx, a->d_vprev, c->red_partials,
-                                 c->red_counters, c->red_out, c->st));
-    VOX8_TRY(launch_tail_select(s, c->red_out, state, c->st));
-    VOX8_TRY(launch_gather_col(nd, a->d_X, a->ldx, state, a->d_vprev, c->st));
-    // eigenvalue sensitivity branch, gated on the violation (state[2])
-    const double* gate = state + 2;
-    VOX8_TRY(launch_quadforms(c->g, c->fK, c->fM, c->d_active, c->dKe, c->dMe, nullptr, nd, 0,
-                              c->d_wts, a->d_vprev, nullptr, c->d_kphi, c->d_mphi, c->red_partials,
-                              c->red_counters, c->d_sums, c->d_tip, c->n_tip, c->tip_m, c->st,
-                              gate, c->pattern != PAT_DENSE ? &c->c64 : nullptr));
-    VOX8_TRY(launch_sens_finish(ne, c->d_active, a->d_rho, a->p, a->e_min, a->mthr, a->mq, c->d_ce,
-                                nullptr, c->d_kphi, c->d_mphi, 0.0, c->d_sums, a->d_dlam, c->st,
-                                state + 1, gate));
-    VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask,
-                           c->d_den, a->d_dlam, a->d_dlam_f, FILTER_FWD, c->st, gate));
-    VOX8_TRY(launch_stats(ne, a->d_design, a->d_sens_f, nullptr, c->red_partials, c->red_counters,
-                          state + 3, c->st, gate));
-    VOX8_TRY(launch_stats(ne, a->d_design, a->d_dlam_f, nullptr, c->red_partials, c->red_counters,
-                          state + 7, c->st, gate));
-    VOX8_TRY(launch_tail_penalty(ne, a->mu, state, a->d_dlam_f, a->d_sens_f, c->st));
-  } else {
-    VOX8_TRY(launch_tail_select(s, nullptr, state, c->st));
   }
-  VOX8_TRY(launch_oc(ne, a->d_design, a->d_x, a->d_sens_f, a->V_e, a->target_volume, a->move,
-                     c->d_oc, c->red_partials, c->red_counters, a->d_xnew, 100, c->st));
-  VOX8_TRY(launch_filter(c->g.nx, c->g.ny, c->g.nz, c->ntap, c->d_toff, c->d_tw, c->d_fmask, c->d_den,
-                         a->d_xnew, a->d_check, FILTER_FWD, c->st));
-  VOX8_TRY(launch_stats(ne, nullptr, a->d_check, a->d_phys_prev, c->red_partials, c->red_counters,
-                        state + 11, c->st));
-  VOX8_TRY(launch_stats(ne, a->d_design, a->d_xnew, nullptr, c->red_partials, c->red_counters,
-                        state + 15, c->st));
-  return sync_read(c, state, h_state, TAIL_STATE);
+  // the per-call values, through pinned memory (the previous call's read
+  // synchronised the stream, so the staging is free)
+  TailPar* hp = c->h_tail_par;
+  *hp = TailPar{};
+  for (int j = 0; j < a->k; ++j) hp->lam[j] = a->h_lam[j];
+  hp->mu = a->mu;
+  hp->X = a->d_X;
+  VOX8_HIP(hipMemcpyAsync(c->d_tail_par, hp, sizeof(TailPar), hipMemcpyHostToDevice, c->st));
+  hipGraphExec_t exec = nullptr;
+  if (c->tail_graphs) {
+    std::vector<unsigned char> key = tail_key(c, a);
+    for (auto& tg : c->tail_cache)
+      if (tg.key == key) exec = tg.exec;
+    if (!exec) {
+      if (c->tail_cache.size() >= 4) {               // least recently used out
+        size_t old = 0;
+        for (size_t i = 1; i < c->tail_cache.size(); ++i)
+          if (c->tail_cache[i].last_use < c->tail_cache[old].last_use) old = i;
+        {
+          HipSerial lock(hip_serial());
+          (void)hipGraphExecDestroy(c->tail_cache[old].exec);
+        }
+        c->tail_cache.erase(c->tail_cache.begin() + old);
+      }
+      if (tail_capture(c, a, &exec)) {
+        vox8_ctx::TailGraph tg;
+        tg.key = std::move(key);
+        tg.exec = exec;
+        c->tail_cache.push_back(std::move(tg));
+      } else if (c->tail_graphs == 2) {
+        set_error("vox8_simp_tail: the runtime refused the graph capture");
+        return ERR_HIP;
+      } else {
+        c->tail_graphs = 0;                           // the runtime refuses: launch directly
+      }
+    }
+  }
+  if (exec) {
+    for (auto& tg : c->tail_cache)
+      if (tg.exec == exec) tg.last_use = ++c->tail_clock;
+    VOX8_HIP(hipGraphLaunch(exec, c->st));
+  } else {
+    VOX8_TRY(tail_launches(c, a, c->st));
+  }
+  return sync_read(c, c->d_tail, h_state, TAIL_STATE);
   API_END
+}
+
+int vox8_set_tail_graph(vox8_ctx* c, int enable) {
+  VOX8_CHECK(c != nullptr, "null context");
+  HipSerial lock(hip_serial());
+  VOX8_CHECK(enable >= 0 && enable <= 2, "enable must be 0, 1 or 2");
+  c->tail_graphs = enable;
+  for (auto& tg : c->tail_cache) (void)hipGraphExecDestroy(tg.exec);
+  c->tail_cache.clear();
+  return OK;
 }
 
 int vox8_effective_mass(vox8_ctx* c, int k, const double* d_X, int64_t ld, int direction,

@@ -24,22 +24,33 @@ int launch_oc(int64_t n, const uint8_t* design, const double* x, const double* d
               double* xnew, int max_steps, hipStream_t st);
 int launch_stats(int64_t n, const uint8_t* mask, const double* a, const double* b, double* partials,
                  unsigned int* counter, double* out, hipStream_t st, const double* gate = nullptr);
+// Xind (may be null): device address holding X, read by the kernel in place
+// of X (a captured launch then follows the pointer stored there).
 int launch_block_dots(int64_t n, int k, const double* X, int64_t ldx, const double* v,
-                      double* partials, unsigned int* counter, double* out, hipStream_t st);
+                      double* partials, unsigned int* counter, double* out, hipStream_t st,
+                      const double* const* Xind = nullptr);
 int launch_axpby(int64_t n, const uint8_t* mask, double a, const double* x, double b, double* y,
                  hipStream_t st);
 // Device-resident loop tail (state layout in vox8_topopt.hip).
 constexpr int TAIL_STATE = 20;
-struct TailSelect {
+// The values of one call that change from iteration to iteration, read by
+// the tail's kernels from device memory, so the tail's launches can be
+// captured once and replayed with new values.
+struct TailPar {
   double lam[16];     // eigenvalues, ascending, clipped at 0
+  double mu;          // frequency-penalty coefficient
+  const double* X;    // the mode block (k column-major vectors)
+};
+struct TailSelect {
   double w_target;    // (2 pi f_target)^2
   int k;              // number of modes (0: no modal constraint)
   int have_prev;      // a previous tracked mode exists
 };
-int launch_tail_select(const TailSelect& s, const double* dots, double* state, hipStream_t st);
-int launch_gather_col(int64_t n, const double* X, int64_t ldx, const double* state, double* v,
+int launch_tail_select(const TailSelect& s, const TailPar* par, const double* dots, double* state,
+                       hipStream_t st);
+int launch_gather_col(int64_t n, const TailPar* par, int64_t ldx, const double* state, double* v,
                       hipStream_t st);
-int launch_tail_penalty(int64_t n, double mu, double* state, const double* x, double* y,
+int launch_tail_penalty(int64_t n, const TailPar* par, double* state, const double* x, double* y,
                         hipStream_t st);
 int launch_compose(int64_t n, const uint8_t* design, const uint8_t* in_arm, const double* filt,
                    double* out, hipStream_t st);

@@ -505,8 +505,10 @@ int launch_stats(int64_t n, const uint8_t* mask, const double* a, const double* 
 __global__ void __launch_bounds__(256)
 k_block_dots(int64_t n, int k, const double* __restrict__ X, int64_t ldx,
              const double* __restrict__ v, double* __restrict__ partials,
-             unsigned int* __restrict__ counter, double* __restrict__ out) {
+             unsigned int* __restrict__ counter, double* __restrict__ out,
+             const double* const* __restrict__ Xind) {
   __shared__ double sh[8], sv[33];
+  if (Xind) X = *Xind;
   double acc[33];
   const int nv = 2 * k + 1;
   for (int i = 0; i < 33; ++i) acc[i] = 0.0;
@@ -529,10 +531,11 @@ k_block_dots(int64_t n, int k, const double* __restrict__ X, int64_t ldx,
 }
 
 int launch_block_dots(int64_t n, int k, const double* X, int64_t ldx, const double* v,
-                      double* partials, unsigned int* counter, double* out, hipStream_t st) {
+                      double* partials, unsigned int* counter, double* out, hipStream_t st,
+                      const double* const* Xind) {
   VOX8_CHECK(k >= 1 && k <= 16, "block_dots: 1..16 vectors");
   hipLaunchKernelGGL(k_block_dots, dim3(nblk(n, 512)), dim3(256), 0, st, n, k, X, ldx, v, partials,
-                     counter, out);
+                     counter, out, Xind);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
@@ -561,8 +564,8 @@ int launch_axpby(int64_t n, const uint8_t* mask, double a, const double* x, doub
 // Mode tracking by MAC against the previous tracked mode (src/topopt.py:178-188,
 // 327-339): dots = {X_j.v, X_j.X_j, v.v} of the eigenvalue-sorted modes; the
 // first maximum wins (numpy argmax); no previous mode -> the lowest.
-__global__ void k_tail_select(TailSelect s, const double* __restrict__ dots,
-                              double* __restrict__ state) {
+__global__ void k_tail_select(TailSelect s, const TailPar* __restrict__ par,
+                              const double* __restrict__ dots, double* __restrict__ state) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int best = 0;
   if (s.have_prev && s.k > 0) {
@@ -573,7 +576,7 @@ __global__ void k_tail_select(TailSelect s, const double* __restrict__ dots,
       if (j == 0 || mac > bm) { bm = mac; best = j; }
     }
   }
-  const double lam1 = s.k > 0 ? s.lam[best] : 0.0;
+  const double lam1 = s.k > 0 ? par->lam[best] : 0.0;
   for (int i = 3; i < TAIL_STATE; ++i) state[i] = 0.0;   // gated stages leave zeros
   state[0] = (double)best;
   state[1] = lam1;
@@ -581,8 +584,9 @@ __global__ void k_tail_select(TailSelect s, const double* __restrict__ dots,
 }
 
 // v = X[:, state[0]] (the tracked mode becomes the next step's reference)
-__global__ void k_gather_col(int64_t n, const double* __restrict__ X, int64_t ldx,
+__global__ void k_gather_col(int64_t n, const TailPar* __restrict__ par, int64_t ldx,
                              const double* __restrict__ state, double* __restrict__ v) {
+  const double* __restrict__ X = par->X;
   const int64_t col = (int64_t)state[0];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -593,8 +597,9 @@ __global__ void k_gather_col(int64_t n, const double* __restrict__ X, int64_t ld
 // violation is positive, y += a x with a = -mu viol (s_c / max(s_m, 1e-30)),
 // s_c and s_m the mean |.| of the filtered compliance and eigenvalue
 // sensitivities over the design (both divided by the design count).
-__global__ void k_tail_penalty(int64_t n, double mu, double* __restrict__ state,
+__global__ void k_tail_penalty(int64_t n, const TailPar* __restrict__ par, double* __restrict__ state,
                                const double* __restrict__ x, double b, double* __restrict__ y) {
+  const double mu = par->mu;
   const double viol = state[2];
   if (!(viol > 0.0)) return;
   const double cnt = state[6];
@@ -606,22 +611,23 @@ __global__ void k_tail_penalty(int64_t n, double mu, double* __restrict__ state,
     y[e] = a * x[e] + b * y[e];
 }
 
-int launch_tail_select(const TailSelect& s, const double* dots, double* state, hipStream_t st) {
-  hipLaunchKernelGGL(k_tail_select, dim3(1), dim3(64), 0, st, s, dots, state);
+int launch_tail_select(const TailSelect& s, const TailPar* par, const double* dots, double* state,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_tail_select, dim3(1), dim3(64), 0, st, s, par, dots, state);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
 
-int launch_gather_col(int64_t n, const double* X, int64_t ldx, const double* state, double* v,
+int launch_gather_col(int64_t n, const TailPar* par, int64_t ldx, const double* state, double* v,
                       hipStream_t st) {
-  hipLaunchKernelGGL(k_gather_col, dim3(nblk(n, 4096)), dim3(256), 0, st, n, X, ldx, state, v);
+  hipLaunchKernelGGL(k_gather_col, dim3(nblk(n, 4096)), dim3(256), 0, st, n, par, ldx, state, v);
   VOX8_HIP(hipGetLastError());
   return OK;
 }
 
-int launch_tail_penalty(int64_t n, double mu, double* state, const double* x, double* y,
+int launch_tail_penalty(int64_t n, const TailPar* par, double* state, const double* x, double* y,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_tail_penalty, dim3(nblk(n, 4096)), dim3(256), 0, st, n, mu, state, x, 1.0, y);
+  hipLaunchKernelGGL(k_tail_penalty, dim3(nblk(n, 4096)), dim3(256), 0, st, n, par, state, x, 1.0, y);
   VOX8_HIP(hipGetLastError());
   return OK;
 }

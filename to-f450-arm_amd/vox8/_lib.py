@@ -78,6 +78,7 @@ SIGNATURES = {
     "vox8_axpby": [_vp, _i64, _vp, _f64, _vp, _f64, _vp],
     "vox8_compose_density": [_vp, _vp, _vp, _vp, _vp],
     "vox8_simp_tail": [_vp, C.POINTER(TailArgs), _pf64],
+    "vox8_set_tail_graph": [_vp, _i32],
     "vox8_effective_mass": [_vp, _i32, _vp, _i64, _i32, _pf64, _pf64, _pf64, _pf64],
     "vox8_trial_mfma_apply": [_vp, _vp, _vp, _i32],
 }
