@@ -222,7 +222,10 @@ int ensure_lobpcg_work(vox8_ctx* c, int k) {
   w.gram_chunks = (int)chunks;
   VOX8_TRY(dalloc(c, &w.gram_partials, (size_t)chunks * 2 * 36 * 36));
   VOX8_TRY(dalloc(c, &w.res, 16));
-  VOX8_TRY(dalloc(c, &w.res_partials, (size_t)1024 * 12));
+  // residual partials: the residual kernel's 1024 blocks x 12, or the product
+  // kernel's workgroups x columns when the residual is formed there
+  w.res_partials_cap = std::max<int64_t>((int64_t)1024 * 12, (int64_t)k * stencil_max_blocks(c->g));
+  VOX8_TRY(dalloc(c, &w.res_partials, (size_t)w.res_partials_cap));
   VOX8_TRY(dalloc(c, &w.counters, 4));
   VOX8_HIP(hipMemsetAsync(w.counters, 0, sizeof(unsigned int) * 4, c->st));
   VOX8_TRY(halloc(c, &w.host, 32));
@@ -272,6 +275,12 @@ int ensure_mg(vox8_ctx* c, int ncols) {
     c->mg_version = c->density_version;
   }
   return OK;
+}
+
+// An environment switch set to "0" (tuning / A-B runs).
+bool env_off(const char* name) {
+  const char* e = getenv(name);
+  return e && e[0] == '0';
 }
 
 int sync_read(vox8_ctx* c, const double* d, double* h, int count) {
@@ -730,7 +739,27 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
   A.g = &c->g;
   A.fixed = c->d_fixed;
   A.dinv = c->dinvK;
-  auto km = [c](const double* x, double* yK, double* yM, int nr, hipStream_t, const int* skip) {
+  auto km = [c](const double* x, double* yK, double* yM, int nr, hipStream_t, const int* skip,
+                const ResidualFuse* rf, bool* fused) {
+    // the residual goes into the structured stencil's epilogue unless a tip
+    // mass is added to M x after that kernel, or the element kernel is dense
+    if (rf && fused && c->pattern != PAT_DENSE && c->stencil_kind == 0 && c->n_tip == 0 &&
+        !env_off("VOX8_FUSED_RESIDUAL")) {
+      Epilogue<double> ep;
+      ep.mode = EPI_LOBRES;
+      ep.lam = rf->lam;
+      ep.conv = rf->conv;
+      ep.dinv = rf->dinv;
+      ep.w = rf->w;
+      ep.ld = c->g.n_dofs;
+      ep.partials = rf->partials;
+      ep.partials_cap = rf->partials_cap;
+      ep.res = rf->res;
+      *fused = true;
+      return launch_apply_fast<double>(c->g, c->pattern, OP_KM, c->c64, c->fK, c->fM, c->d_fixed, x,
+                                       c->g.n_dofs, yK, yM, c->g.n_dofs, nr, c->st, DotFuse(), ep);
+    }
+    if (fused) *fused = false;
     return apply_op<double>(c, OP_KM, x, c->g.n_dofs, yK, yM, c->g.n_dofs, nr, DotFuse(), skip);
   };
   LobpcgResult r{};

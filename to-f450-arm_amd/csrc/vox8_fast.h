@@ -50,8 +50,12 @@ int fast_coefficients(const double* Ke, const double* Me, FastCoef<double>* c64,
 // x_0 = 0: the input is b and x_1 is formed on load.  EPI_X takes x_{k-1}
 // from `xprev` (prev 1), as c0 D^-1 b (prev 2: the step after EPI_X0), or
 // drops the c1 term (prev 0: the first step from a non-zero iterate).
+// EPI_LOBRES (FP64 K and M products, LOBPCG): besides K x and M x, the
+// residual r = K x - lam_col M x of each column: W = D^-1 r (r without dinv),
+// zero for the columns flagged in conv, and per-workgroup partial sums of
+// |r|^2 (partials[col * tiles + tile]).
 enum EpiMode : int { EPI_PLAIN = 0, EPI_RESID = 1, EPI_CHEB = 2, EPI_CHEB0 = 3, EPI_X0 = 4,
-                     EPI_X = 5 };
+                     EPI_X = 5, EPI_LOBRES = 6 };
 
 // Node-local epilogue of a K product (multigrid smoothing), vectors with ld.
 template <typename T>
@@ -69,6 +73,12 @@ struct Epilogue {
   int prev = 0;              // EPI_X: where x_{k-1} comes from (see EpiMode)
   double* out64 = nullptr;   // EPI_X (FP32 structured stencil): x_{k+1} stored as FP64 here
                              // instead of into xs
+  const double* lam = nullptr;   // EPI_LOBRES: per-column eigenvalue estimate
+  const int* conv = nullptr;     //   per-column lock flags (W written as zero), may be null
+  double* w = nullptr;           //   W (ld), the residual's output
+  double* partials = nullptr;    //   |r|^2 per (column, workgroup)
+  double* res = nullptr;         //   |r|^2 per column (summed in a fixed order after the launch)
+  int64_t partials_cap = 0;      //   doubles in partials
 };
 
 
@@ -88,6 +98,9 @@ int stencil_tune_fp32(const Grid& g, int pattern, const FastCoef<float>& C, cons
                       const uint8_t* fixed, const float* dinv, float* d, float* r, float* xs,
                       float* dnew, int64_t ld, int ncols, hipStream_t st);
 bool stencil_tuned_fp32(const Grid& g, int ncols);
+// Most workgroups one column of a structured-stencil launch can have (any
+// tile rows and z chunk): the size of per-workgroup partial sums.
+int64_t stencil_max_blocks(const Grid& g);
 
 template <typename T>
 int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, const T* fK,

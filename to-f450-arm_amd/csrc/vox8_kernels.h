@@ -77,6 +77,7 @@ __device__ __forceinline__ void sum_partials(const double* partials, int nb, dou
 // Finishing kernel of a fused K p . p product: scal[col].pq = sum of the nb
 // partials of column col (partials + col * nb); grid (1, ncols), 256 threads.
 int launch_pq_finish(const DotFuse& fuse, int nb, int ncols, hipStream_t st);
+int launch_partials_finish(const double* partials, int nb, int ncols, double* out, hipStream_t st);
 
 // "Last block finalises" grid reduction of NV values per block.  Each block
 // passes its block sums in `vals` (thread 0 only reads them); partial sums go to

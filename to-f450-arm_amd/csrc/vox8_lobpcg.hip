@@ -586,13 +586,20 @@ static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
   const double* lam = C + MMAX * 12;
   VOX8_K_SWITCH(k, launch_combine, w, nb, C, st);
   VOX8_HIP(hipGetLastError());
-  VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st, nullptr));
+  // the residual with the products of X' where the product kernel can form
+  // it (W is free: P' = [W P] C was formed above)
+  ResidualFuse rf{lam, w.conv, dinv, w.S + (int64_t)k * w.n, w.res_partials, w.res_partials_cap,
+                  w.res};
+  bool fused = false;
+  VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st, nullptr, &rf, &fused));
   if (nb > 1) {
     const int64_t off = (int64_t)2 * k * w.n;
-    VOX8_TRY(applyKM(w.S + off, w.AS + off, w.BS + off, k, st, w.conv));
+    VOX8_TRY(applyKM(w.S + off, w.AS + off, w.BS + off, k, st, w.conv, nullptr, nullptr));
   }
-  VOX8_K_SWITCH(k, launch_residual, w, lam, dinv, st);
-  VOX8_HIP(hipGetLastError());
+  if (!fused) {
+    VOX8_K_SWITCH(k, launch_residual, w, lam, dinv, st);
+    VOX8_HIP(hipGetLastError());
+  }
   if (tol2 > 0.0) {
     VOX8_K_SWITCH(k, launch_lock, w, tol2, st);
     VOX8_HIP(hipGetLastError());
@@ -644,7 +651,7 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
     VOX8_HIP(hipGetLastError());
   }
   // initial Rayleigh-Ritz on X alone
-  VOX8_TRY(applyKM(X, w.AS, w.BS, k, st, nullptr));
+  VOX8_TRY(applyKM(X, w.AS, w.BS, k, st, nullptr, nullptr, nullptr));
   const double* Tdinv = M ? nullptr : A.dinv;
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 12, w.conv);
   VOX8_HIP(hipGetLastError());
@@ -682,7 +689,7 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
   };
   trace();
   while (!all && it < maxiter) {
-    VOX8_TRY(applyKM(W, AW, BW, k, st, w.conv));   // locked pairs: W, KW, MW stay zero
+    VOX8_TRY(applyKM(W, AW, BW, k, st, w.conv, nullptr, nullptr));   // locked pairs: W, KW, MW stay zero
     const int m = have_p ? 3 * k : 2 * k;
     // the Rayleigh-Ritz kernel restarts without P on the device when needed
     VOX8_TRY(gram_and_rr(w, m, k, st, drop_tol));

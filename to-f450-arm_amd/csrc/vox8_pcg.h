@@ -40,9 +40,24 @@ struct PcgWork {
 template <typename T>
 using Precond = std::function<int(const T* r, T* z, int nrhs, hipStream_t st, const int* skip)>;
 
+// The LOBPCG residual formed with the K and M products of X: W = D^-1 r
+// (dinv null: W = r), r = K x_j - lam_j M x_j, zero for the columns flagged
+// in conv; |r_j|^2 into res[j].
+struct ResidualFuse {
+  const double* lam;
+  const int* conv;
+  const double* dinv;
+  double* w;               // nrhs columns, ld n_dofs
+  double* partials;        // partials_cap doubles
+  int64_t partials_cap;
+  double* res;
+};
+
 // K and M products of nrhs columns; skip as above (those outputs untouched).
+// rf (may be null): also form that residual if the product can (*fused is
+// then true; otherwise the caller forms it).
 using ApplyKM = std::function<int(const double* x, double* yK, double* yM, int nrhs, hipStream_t st,
-                                  const int* skip)>;
+                                  const int* skip, const ResidualFuse* rf, bool* fused)>;
 
 struct PcgResult {
   int iters;      // CG iterations (summed over inner solves for the mixed path)
@@ -73,6 +88,7 @@ struct LobpcgWork {
   double* res = nullptr;      // device residual norms^2 [k]
   double* host = nullptr;     // pinned mirror of lam|res|status
   double* res_partials = nullptr;
+  int64_t res_partials_cap = 0;   // doubles in res_partials
   unsigned int* counters = nullptr;
   int* conv = nullptr;        // device: sticky convergence flags [12] (soft locking)
   double* guard = nullptr;    // last solve's guard vectors [gmax][n] (warm start of the next)

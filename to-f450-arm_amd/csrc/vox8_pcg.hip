@@ -232,6 +232,22 @@ k_pq_finish(PcgScalars* __restrict__ scal, const double* __restrict__ partials, 
   if (threadIdx.x == 0) scal[col].pq = tot;
 }
 
+// Column sums of per-workgroup partials (partials[col * nb + b]), fixed order.
+__global__ void __launch_bounds__(256)
+k_partials_finish(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+  __shared__ double sh[8];
+  const int col = blockIdx.y;
+  double tot;
+  sum_partials<1>(partials + (int64_t)col * nb, nb, &tot, sh);
+  if (threadIdx.x == 0) out[col] = tot;
+}
+
+int launch_partials_finish(const double* partials, int nb, int ncols, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_partials_finish, dim3(1, ncols), dim3(256), 0, st, partials, nb, out);
+  VOX8_HIP(hipGetLastError());
+  return OK;
+}
+
 int launch_pq_finish(const DotFuse& fuse, int nb, int ncols, hipStream_t st) {
   hipLaunchKernelGGL(k_pq_finish, dim3(1, ncols), dim3(256), 0, st, fuse.scal,
                      (const double*)fuse.partials, nb);

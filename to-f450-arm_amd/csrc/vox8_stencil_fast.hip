@@ -320,11 +320,15 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const int ntiles = nb / ncols;
   if (FUSE && fuse.scal[col].done) return;
   if (ep.skip && ep.skip[col]) return;
-  x += col * ldx;
-  if (DO_K && MODE == EPI_PLAIN) yK += col * ldy;
-  if (DO_M) yM += col * ldy;
   constexpr bool XF = MODE == EPI_X0 || MODE == EPI_X;          // three-term iterate form
-  if (MODE != EPI_PLAIN) {
+  constexpr bool LR = MODE == EPI_LOBRES;
+  x += col * ldx;
+  if (DO_K && (MODE == EPI_PLAIN || LR)) yK += col * ldy;
+  if (DO_M) yM += col * ldy;
+  const double lamc = LR ? ep.lam[col] : 0.0;
+  const bool lockd = LR && ep.conv && ep.conv[col];
+  if (LR) ep.w += col * ep.ld;
+  if (MODE != EPI_PLAIN && !LR) {
     if (!XF) {
       ep.r += col * ep.ld;
       if (ep.dnew) ep.dnew += col * ep.ld;
@@ -364,15 +368,16 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const Rsrc rfix = make_rsrc(fixed, (uint32_t)g.n_nodes);
   const Rsrc rfk = make_rsrc(fK, DO_K ? ebytes : 0u);
   const Rsrc rfm = make_rsrc(fM, DO_M ? ebytes : 0u);
-  const Rsrc rdi = make_rsrc(ep.dinv, MODE != EPI_PLAIN ? vbytes : 0u);
+  const Rsrc rdi = make_rsrc(ep.dinv, (MODE != EPI_PLAIN && ep.dinv) ? vbytes : 0u);
   const Rsrc rb = make_rsrc((MODE == EPI_RESID || MODE == EPI_X) ? ep.b : (MODE == EPI_CHEB ? ep.r : x),
                             MODE == EPI_PLAIN ? 0u : vbytes);        // epilogue input a
   const Rsrc rxs = make_rsrc(ep.xs, (MODE == EPI_CHEB || MODE == EPI_CHEB0 || (XF && !ep.out64))
                                         ? vbytes : 0u);
   const Rsrc rxp = make_rsrc(ep.xprev, (MODE == EPI_X && ep.prev == 1) ? vbytes : 0u);
   const Rsrc r64 = make_rsrc(ep.out64, (MODE == EPI_X && ep.out64) ? 2u * vbytes : 0u);
-  const Rsrc ry1 = make_rsrc(MODE == EPI_PLAIN ? yK : ep.r, (DO_K && !XF) ? vbytes : 0u);
-  const Rsrc ry2 = make_rsrc(ep.dnew, (MODE != EPI_PLAIN && !XF && ep.dnew) ? vbytes : 0u);
+  const Rsrc ry1 = make_rsrc((MODE == EPI_PLAIN || LR) ? yK : ep.r, (DO_K && !XF) ? vbytes : 0u);
+  const Rsrc rw = make_rsrc(ep.w, LR ? vbytes : 0u);
+  const Rsrc ry2 = make_rsrc(ep.dnew, (MODE != EPI_PLAIN && !XF && !LR && ep.dnew) ? vbytes : 0u);
   const Rsrc rym = make_rsrc(yM, DO_M ? vbytes : 0u);
 
   // plane staging: thread loads nodes q = t + i NT of the (PX x PY) plane;
@@ -442,10 +447,8 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     const uint32_t nn = nxy + plane * (uint32_t)L;
     o.off = ok ? 3u * (uint32_t)sizeof(T) * nn : OOB;
     o.fnode = ld_u8(rfix, ok ? nn : OOB);
-    if (MODE != EPI_PLAIN) {
-      ld3(rb, o.off, o.a);
-      ld3(rdi, o.off, o.di);
-    }
+    if (MODE != EPI_PLAIN && !LR) ld3(rb, o.off, o.a);
+    if (MODE != EPI_PLAIN) ld3(rdi, o.off, o.di);
     if (MODE == EPI_CHEB) ld3(rxs, o.off, o.xs);
     if (MODE == EPI_X) ld3(rxp, o.off, o.xs);
   };
@@ -555,6 +558,8 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
         vK[c] = val;
         if (MODE == EPI_PLAIN) {
           if (FUSE) dot += cur.off != OOB ? (double)val * (double)xin[c] : 0.0;
+        } else if (LR) {
+          // formed below, once M x is known
         } else if (MODE == EPI_RESID) {
           const T r = fx ? T(0) : cur.a[c] - val;
           o1[c] = r;
@@ -580,10 +585,16 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
         }
       }
       if (DO_M) vM[c] = fx ? T(0) : bM[c];
+      if constexpr (LR && DO_K && DO_M) {
+        const double r = (double)vK[c] - lamc * (double)vM[c];
+        dot += cur.off != OOB ? r * r : 0.0;
+        o1[c] = lockd ? T(0) : (ep.dinv ? (T)(cur.di[c] * r) : (T)r);
+      }
     }
     if (DO_K) {
-      if (MODE == EPI_PLAIN) {
+      if (MODE == EPI_PLAIN || LR) {
         st3(ry1, cur.off, vK);
+        if (LR) st3(rw, cur.off, o1);
       } else if (XF) {
         if (MODE == EPI_X && ep.out64) {
           const double w[3] = {(double)o3[0], (double)o3[1], (double)o3[2]};
@@ -610,9 +621,9 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
       __syncthreads();
     }
   }
-  if (FUSE) {
+  if (FUSE || LR) {
     const double v = block_sum(dot, sh);
-    write_partials<1>(&v, fuse.partials + (int64_t)col * ntiles, tile);
+    write_partials<1>(&v, (FUSE ? fuse.partials : ep.partials) + (int64_t)col * ntiles, tile);
   }
 }
 
@@ -653,6 +664,10 @@ static int fast_zchunk(const Grid& g, int ty, int ncols) {
 
 static int fast_blocks(const Grid& g, int ty, int zc) {
   return tiles_x(g) * tiles_y(g, ty) * ((g.NZ + zc - 1) / zc);
+}
+
+int64_t stencil_max_blocks(const Grid& g) {
+  return (int64_t)std::max(fast_blocks(g, 8, 1), fast_blocks(g, 12, 1));
 }
 
 // ---- shape autotuner (FP32) ----------------------------------------------
@@ -926,7 +941,13 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     return ERR_ARG;
   }
   const bool fz = fuse.scal != nullptr;
-  if (ep.mode != EPI_PLAIN && (op != OP_K || fz)) {
+  if (ep.mode == EPI_LOBRES) {
+    if (sizeof(T) != 8 || op != OP_KM || fz || !ep.lam || !ep.w || !ep.partials ||
+        (int64_t)nrhs * stencil_max_blocks(g) > ep.partials_cap) {
+      set_error("residual epilogue: FP64 K and M products with lam, w and partials");
+      return ERR_ARG;
+    }
+  } else if (ep.mode != EPI_PLAIN && (op != OP_K || fz)) {
     set_error("stencil epilogue modes apply to K alone");
     return ERR_ARG;
   }
@@ -967,7 +988,9 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
     else if (fz) VOX8_LAUNCH(PAT, true, false, true, EPI_PLAIN);                         \
     else VOX8_LAUNCH(PAT, true, false, false, EPI_PLAIN);                               \
   } else if (op == OP_M) VOX8_LAUNCH(PAT, false, true, false, EPI_PLAIN);                \
-  else VOX8_LAUNCH(PAT, true, true, false, EPI_PLAIN);
+  else if (ep.mode == EPI_LOBRES) {                                                     \
+    if constexpr (sizeof(T) == 8) VOX8_LAUNCH(PAT, true, true, false, EPI_LOBRES);      \
+  } else VOX8_LAUNCH(PAT, true, true, false, EPI_PLAIN);
     if (pattern == PAT_CUBE) { VOX8_PAT(PAT_CUBE) }
     else if (pattern == PAT_ORTHO) { VOX8_PAT(PAT_ORTHO) }
     else { VOX8_PAT(PAT_BLOCK) }
@@ -978,12 +1001,15 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
   };
   StencilShape shp = stencil_shape<T>(g, nrhs);
   if constexpr (sizeof(T) == 8) {
-    if (!fz && env_int("VOX8_ZCHUNK", 0) <= 0)
+    // the fused sums (PCG p.Kp, LOBPCG residual norms) are per workgroup:
+    // their shape keeps the fixed rule, so the bits do not depend on timing
+    if (!fz && ep.mode != EPI_LOBRES && env_int("VOX8_ZCHUNK", 0) <= 0)
       VOX8_TRY(tuned_zchunk64(g, op, nrhs, st, launch, &shp.zc));
   }
   const int nb = launch(shp.ty, shp.zc);
   VOX8_HIP(hipGetLastError());
   if (fz) VOX8_TRY(launch_pq_finish(fuse, nb, nrhs, st));
+  if (ep.mode == EPI_LOBRES && ep.res) VOX8_TRY(launch_partials_finish(ep.partials, nb, nrhs, ep.res, st));
   return OK;
 }
 
