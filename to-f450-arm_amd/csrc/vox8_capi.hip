@@ -750,7 +750,8 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
       ep.lam = rf->lam;
       ep.conv = rf->conv;
       ep.dinv = rf->dinv;
-      ep.w = rf->w;
+      ep.w = rf->w32 ? nullptr : rf->w;
+      ep.w32 = rf->w32;
       ep.ld = c->g.n_dofs;
       ep.partials = rf->partials;
       ep.partials_cap = rf->partials_cap;
@@ -768,7 +769,10 @@ int vox8_lobpcg(vox8_ctx* c, int k, double tol, int maxiter, int prec, const dou
     Precond<double> M = [c](const double* rr, double* z, int nr, hipStream_t st, const int* skip) {
       return mg_apply(c->mg, rr, z, nr, st, skip);
     };
-    VOX8_TRY(lobpcg_fp64(A, km, k, guard, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed, &M));
+    // the residual goes straight into the cycle's FP32 right-hand side
+    float* m32 = env_off("VOX8_RESIDUAL_FP32") ? nullptr : c->mg.lv[0].b;
+    VOX8_TRY(lobpcg_fp64(A, km, k, guard, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed, &M,
+                         m32));
   } else {
     VOX8_TRY(lobpcg_fp64(A, km, k, guard, tol, maxiter, d_X0, d_X, h_lam, c->lw, c->st, &r, seed));
   }

@@ -76,6 +76,7 @@ struct Epilogue {
   const double* lam = nullptr;   // EPI_LOBRES: per-column eigenvalue estimate
   const int* conv = nullptr;     //   per-column lock flags (W written as zero), may be null
   double* w = nullptr;           //   W (ld), the residual's output
+  float* w32 = nullptr;          //   instead of w: W in FP32 (ld), a multigrid cycle's input
   double* partials = nullptr;    //   |r|^2 per (column, workgroup)
   double* res = nullptr;         //   |r|^2 per column (summed in a fixed order after the launch)
   int64_t partials_cap = 0;      //   doubles in partials

@@ -580,18 +580,22 @@ static void launch_residual(const LobpcgWork& w, const double* lam, const double
 // X' (and P') from the Ritz coefficients, their exact K and M products by
 // the stencil, residual; tol2 > 0 also soft-locks converged pairs.  Pairs
 // locked earlier skip the P products (their P, KP, MP are zeroed by the lock).
+// w32 (may be null): the preconditioner's FP32 input, where the residual goes
+// instead of W when the product forms it (*in32 is then true).
 static int ritz_step(LobpcgWork& w, int nb, int k, const double* dinv,
-                     const ApplyKM& applyKM, hipStream_t st, double tol2) {
+                     const ApplyKM& applyKM, hipStream_t st, double tol2, float* w32 = nullptr,
+                     bool* in32 = nullptr) {
   const double* C = w.G + 2 * MMAX * MMAX;
   const double* lam = C + MMAX * 12;
   VOX8_K_SWITCH(k, launch_combine, w, nb, C, st);
   VOX8_HIP(hipGetLastError());
   // the residual with the products of X' where the product kernel can form
   // it (W is free: P' = [W P] C was formed above)
-  ResidualFuse rf{lam, w.conv, dinv, w.S + (int64_t)k * w.n, w.res_partials, w.res_partials_cap,
-                  w.res};
+  ResidualFuse rf{lam, w.conv, dinv, w.S + (int64_t)k * w.n, w32, w.res_partials,
+                  w.res_partials_cap, w.res};
   bool fused = false;
   VOX8_TRY(applyKM(w.S, w.AS, w.BS, k, st, nullptr, &rf, &fused));
+  if (in32) *in32 = fused && w32;
   if (nb > 1) {
     const int64_t off = (int64_t)2 * k * w.n;
     VOX8_TRY(applyKM(w.S + off, w.AS + off, w.BS + off, k, st, w.conv, nullptr, nullptr));
@@ -620,7 +624,7 @@ static int read_state(LobpcgWork& w, int k, hipStream_t st) {
 int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
                 int kw, int guard, double tol, int maxiter, const double* X0, double* Xout,
                 double* lam_out, LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
-                const Precond<double>* M) {
+                const Precond<double>* M, float* M32) {
   const int k = kw + guard;                     // iterated block: wanted pairs + guard columns
   VOX8_CHECK(kw >= 1 && guard >= 0 && guard <= LOBPCG_GUARD_MAX && k <= 12 && k <= w.kmax,
              "lobpcg: block size must be 1..12");
@@ -656,9 +660,14 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, 12, w.conv);
   VOX8_HIP(hipGetLastError());
   const double tol2 = tol * tol;
+  // with a preconditioner reading FP32 (M32), the residual of a fused
+  // product goes straight there: W is then written only by the preconditioner
+  // (locked pairs' W stays the zero the lock wrote)
+  float* w32 = M ? M32 : nullptr;
+  bool in32 = false;
   VOX8_TRY(gram_and_rr(w, k, k, st, drop_tol));
-  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st, tol2));
-  if (M) VOX8_TRY((*M)(W, W, k, st, w.conv));
+  VOX8_TRY(ritz_step(w, 1, k, Tdinv, applyKM, st, tol2, w32, &in32));
+  if (M) VOX8_TRY((*M)(in32 ? nullptr : W, W, k, st, w.conv));
   VOX8_TRY(read_state(w, k, st));
   VOX8_CHECK(w.host[12] == 0.0, "lobpcg: linearly dependent initial block");
   std::vector<int> conv(kw, 0);
@@ -693,8 +702,8 @@ int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
     const int m = have_p ? 3 * k : 2 * k;
     // the Rayleigh-Ritz kernel restarts without P on the device when needed
     VOX8_TRY(gram_and_rr(w, m, k, st, drop_tol));
-    VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st, tol2));
-    if (M) VOX8_TRY((*M)(W, W, k, st, w.conv));
+    VOX8_TRY(ritz_step(w, m / k, k, Tdinv, applyKM, st, tol2, w32, &in32));
+    if (M) VOX8_TRY((*M)(in32 ? nullptr : W, W, k, st, w.conv));
     have_p = true;
     ++it;
     VOX8_TRY(read_state(w, k, st));

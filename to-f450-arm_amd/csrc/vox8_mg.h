@@ -134,7 +134,9 @@ int mg_reserve(MultiGrid& mg, int ncols, const MGAlloc& al);
 int mg_copy_operators(MultiGrid& dst, const MultiGrid& src, hipStream_t st);
 // z = cycle(r) (V or W, see MultiGrid::gamma) for ncols FP64 column-major
 // vectors (ld = fine n_dofs).  skip (device, per column, may be null):
-// non-zero columns are left untouched (z keeps its contents).
+// non-zero columns are left untouched (z keeps its contents).  r == nullptr:
+// the input is already in the finest level's FP32 right-hand side lv[0].b
+// (written there by the caller's last kernel), so no conversion runs.
 int mg_apply(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,
              const int* skip = nullptr);
 

@@ -961,9 +961,11 @@ static int mg_cycle(MultiGrid& mg, int l, int ncols, hipStream_t st, const int* 
 static int mg_apply_launches(MultiGrid& mg, const double* r, double* z, int ncols, hipStream_t st,
                              const int* skip) {
   MGLevel& F = mg.lv[0];
-  hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b,
-                     skip);
-  VOX8_HIP(hipGetLastError());
+  if (r) {
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(nb(F.g.n_dofs), ncols), dim3(256), 0, st, F.g.n_dofs, r, F.b,
+                       skip);
+    VOX8_HIP(hipGetLastError());
+  }
   // With a coarser level below, the finest level's cycle ends with a
   // post-smoothing sweep, whose last step writes z directly.
   const bool fused_out = mg.lv.size() > 1;

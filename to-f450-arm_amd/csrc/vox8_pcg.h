@@ -48,6 +48,7 @@ struct ResidualFuse {
   const int* conv;
   const double* dinv;
   double* w;               // nrhs columns, ld n_dofs
+  float* w32;              // instead of w (may be null): W in FP32, ld n_dofs
   double* partials;        // partials_cap doubles
   int64_t partials_cap;
   double* res;
@@ -105,9 +106,11 @@ constexpr int LOBPCG_GUARD_MAX = 2;
 
 // k wanted pairs, iterated with `guard` extra columns (k + guard <= 12); only
 // the k lowest are tested for convergence and returned.
+// M32 (with M, may be null): where M reads its input in FP32 when called
+// with r == nullptr; the residual is then written there directly.
 int lobpcg_fp64(const PcgOps<double>& A, const ApplyKM& applyKM,
                 int k, int guard, double tol, int maxiter, const double* X0, double* Xout,
                 double* lam_out, LobpcgWork& w, hipStream_t st, LobpcgResult* res, uint64_t seed,
-                const Precond<double>* M = nullptr);
+                const Precond<double>* M = nullptr, float* M32 = nullptr);
 
 }  // namespace vox8

@@ -327,7 +327,10 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   if (DO_M) yM += col * ldy;
   const double lamc = LR ? ep.lam[col] : 0.0;
   const bool lockd = LR && ep.conv && ep.conv[col];
-  if (LR) ep.w += col * ep.ld;
+  if (LR) {
+    if (ep.w32) ep.w32 += col * ep.ld;
+    else ep.w += col * ep.ld;
+  }
   if (MODE != EPI_PLAIN && !LR) {
     if (!XF) {
       ep.r += col * ep.ld;
@@ -376,7 +379,7 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const Rsrc rxp = make_rsrc(ep.xprev, (MODE == EPI_X && ep.prev == 1) ? vbytes : 0u);
   const Rsrc r64 = make_rsrc(ep.out64, (MODE == EPI_X && ep.out64) ? 2u * vbytes : 0u);
   const Rsrc ry1 = make_rsrc((MODE == EPI_PLAIN || LR) ? yK : ep.r, (DO_K && !XF) ? vbytes : 0u);
-  const Rsrc rw = make_rsrc(ep.w, LR ? vbytes : 0u);
+  const Rsrc rw = LR && ep.w32 ? make_rsrc(ep.w32, vbytes / 2u) : make_rsrc(ep.w, LR ? vbytes : 0u);
   const Rsrc ry2 = make_rsrc(ep.dnew, (MODE != EPI_PLAIN && !XF && !LR && ep.dnew) ? vbytes : 0u);
   const Rsrc rym = make_rsrc(yM, DO_M ? vbytes : 0u);
 
@@ -594,7 +597,12 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
     if (DO_K) {
       if (MODE == EPI_PLAIN || LR) {
         st3(ry1, cur.off, vK);
-        if (LR) st3(rw, cur.off, o1);
+        if (LR && ep.w32) {
+          const float w3[3] = {(float)o1[0], (float)o1[1], (float)o1[2]};
+          st3(rw, cur.off == OOB ? OOB : cur.off / 2u, w3);
+        } else if (LR) {
+          st3(rw, cur.off, o1);
+        }
       } else if (XF) {
         if (MODE == EPI_X && ep.out64) {
           const double w[3] = {(double)o3[0], (double)o3[1], (double)o3[2]};
@@ -942,7 +950,7 @@ int launch_apply_fast(const Grid& g, int pattern, int op, const FastCoef<T>& C, 
   }
   const bool fz = fuse.scal != nullptr;
   if (ep.mode == EPI_LOBRES) {
-    if (sizeof(T) != 8 || op != OP_KM || fz || !ep.lam || !ep.w || !ep.partials ||
+    if (sizeof(T) != 8 || op != OP_KM || fz || !ep.lam || !(ep.w || ep.w32) || !ep.partials ||
         (int64_t)nrhs * stencil_max_blocks(g) > ep.partials_cap) {
       set_error("residual epilogue: FP64 K and M products with lam, w and partials");
       return ERR_ARG;
