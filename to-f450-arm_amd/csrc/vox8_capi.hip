@@ -697,8 +697,10 @@ int vox8_pcg(vox8_ctx* c, const double* d_F, double* d_U, int nrhs, int64_t ld, 
     Precond<double> M = [c](const double* r, double* z, int nr, hipStream_t st, const int* skip) {
       return mg_apply(c->mg, r, z, nr, st, skip);
     };
+    // the residual update writes the cycle's FP32 right-hand side too
+    float* m32 = env_off("VOX8_RESIDUAL_FP32") ? nullptr : c->mg.lv[0].b;
     VOX8_TRY(pcg_fp64(A64, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, c->w64, c->st,
-                      res.data(), &M));
+                      res.data(), &M, m32));
   } else {
     VOX8_CHECK(prec == VOX8_PREC_FP64, "prec must be FP64 (0) or MIXED (1)");
     VOX8_TRY(pcg_fp64(A64, d_F, ld, d_U, x0_given != 0, nrhs, rtol, maxiter, c->w64, c->st,

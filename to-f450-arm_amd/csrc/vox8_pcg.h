@@ -68,9 +68,11 @@ struct PcgResult {
 };
 
 // Batched PCG; M == nullptr: Jacobi (A.dinv) fused into the vector kernels.
+// M32 (with M, may be null): where M reads its input in FP32 when called
+// with r == nullptr; the residual update writes it there as well.
 int pcg_fp64(const PcgOps<double>& A, const double* F, int64_t ldf, double* X, bool x0_given,
              int nrhs, double rtol, int maxiter, PcgWork<double>& w, hipStream_t st,
-             PcgResult* res, const Precond<double>* M = nullptr);
+             PcgResult* res, const Precond<double>* M = nullptr, float* M32 = nullptr);
 int convert_to_float(const double* a, float* b, int64_t n, hipStream_t st);
 
 // ---- LOBPCG ----

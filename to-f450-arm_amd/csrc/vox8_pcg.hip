@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256)
 k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
            T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ q, int64_t ld,
            PcgScalars* __restrict__ scal, int maxiter, double* __restrict__ partials,
-           unsigned int* __restrict__ counters) {
+           unsigned int* __restrict__ counters, float* __restrict__ r32) {
   __shared__ double sh[8];
   const int col = blockIdx.y;
   const PcgScalars s = scal[col];
@@ -120,6 +120,7 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
   double rr = 0.0, rz = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   x += col * ld; r += col * ld; p += col * ld; q += col * ld;
+  if (r32) r32 += col * ld;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n_dofs; i += 4 * stride) {
     T xv[4], pv[4], rv[4], qv[4], dv[4];
@@ -135,6 +136,7 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
       x[j] = (T)((double)xv[u] + alpha * (double)pv[u]);
       const T rn = (T)((double)rv[u] - alpha * (double)qv[u]);
       r[j] = rn;
+      if (r32) r32[j] = (float)rn;
       rr += (double)rn * (double)rn;
       if (RZ) rz += (double)rn * (double)dv[u] * (double)rn;
     }
@@ -143,6 +145,7 @@ k_pcg_step(int64_t n_dofs, const T* __restrict__ dinv, T* __restrict__ x,
     x[i] = (T)((double)x[i] + alpha * (double)p[i]);
     const T rn = (T)((double)r[i] - alpha * (double)q[i]);
     r[i] = rn;
+    if (r32) r32[i] = (float)rn;
     rr += (double)rn * (double)rn;
     if (RZ) rz += (double)rn * (double)dinv[i] * (double)rn;
   }
@@ -304,7 +307,7 @@ int pcg_reduce_capacity_blocks() { return 1024; }
 template <typename T>
 static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const T* x0_or_null,
                   bool x0_given, int nrhs, double rtol, int maxiter, PcgWork<T>& w,
-                  hipStream_t st, int chunk0, const Precond<T>* M) {
+                  hipStream_t st, int chunk0, const Precond<T>* M, float* M32 = nullptr) {
   const Grid& g = *A.g;
   const int64_t n = g.n_dofs;
   const unsigned nb = vec_blocks(n);
@@ -331,6 +334,7 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
   // residual reduction per iteration (plus one), doubling only while no
   // rate is known yet.
   int launched = 0, chunk = chunk0;
+  bool r32_ok = false;
   std::vector<double> rr_seen(nrhs, -1.0);
   std::vector<int> it_seen(nrhs, 0);
   while (launched < maxiter) {
@@ -340,12 +344,14 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
         hipLaunchKernelGGL((k_pcg_dir<T>), grid, blk, 0, st, n, A.dinv, w.r, w.p, n, w.scal, maxiter);
         VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
         hipLaunchKernelGGL((k_pcg_step<T, true>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
-                           w.scal, maxiter, w.partials, w.counters);
+                           w.scal, maxiter, w.partials, w.counters, (float*)nullptr);
         hipLaunchKernelGGL(k_pcg_step_finish<true>, dim3(1, nrhs), blk, 0, st, w.scal, maxiter,
                            (const double*)w.partials, (int)nb);
       } else {
         hipLaunchKernelGGL(k_pcg_check, dim3(1, nrhs), dim3(1), 0, st, w.scal, maxiter, w.skip);
-        VOX8_TRY((*M)(w.r, w.x, nrhs, st, w.skip));
+        // after the first step the step kernel has written the residual in
+        // FP32 where the preconditioner reads it (M32), so M skips its conversion
+        VOX8_TRY((*M)(r32_ok ? nullptr : w.r, w.x, nrhs, st, w.skip));
         hipLaunchKernelGGL((k_pcg_rz<T>), grid, blk, 0, st, n, (const T*)w.r, (const T*)w.x, n,
                            w.scal, maxiter, w.partials, w.counters);
         hipLaunchKernelGGL(k_pcg_rz_finish, dim3(1, nrhs), blk, 0, st, w.scal, maxiter,
@@ -354,7 +360,8 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
                            (const PcgScalars*)w.scal, maxiter);
         VOX8_TRY(A.apply(w.p, n, w.q, n, nrhs, st, fuse));
         hipLaunchKernelGGL((k_pcg_step<T, false>), grid, blk, 0, st, n, A.dinv, x, w.r, w.p, w.q, n,
-                           w.scal, maxiter, w.partials, w.counters);
+                           w.scal, maxiter, w.partials, w.counters, M32);
+        r32_ok = M32 != nullptr;
         hipLaunchKernelGGL(k_pcg_step_finish<false>, dim3(1, nrhs), blk, 0, st, w.scal, maxiter,
                            (const double*)w.partials, (int)nb);
       }
@@ -393,8 +400,9 @@ static int run_cg(const PcgOps<T>& A, const double* F, int64_t ldf, T* x, const 
 
 int pcg_fp64(const PcgOps<double>& A, const double* F, int64_t ldf, double* X, bool x0_given,
              int nrhs, double rtol, int maxiter, PcgWork<double>& w, hipStream_t st,
-             PcgResult* res, const Precond<double>* M) {
-  VOX8_TRY(run_cg<double>(A, F, ldf, X, X, x0_given, nrhs, rtol, maxiter, w, st, M ? 2 : 8, M));
+             PcgResult* res, const Precond<double>* M, float* M32) {
+  VOX8_TRY(run_cg<double>(A, F, ldf, X, X, x0_given, nrhs, rtol, maxiter, w, st, M ? 2 : 8, M,
+                          M ? M32 : nullptr));
   for (int j = 0; j < nrhs; ++j) {
     const PcgScalars& s = w.host_scal[j];
     res[j].iters = s.iters;
