@@ -568,10 +568,12 @@ int gal_apply(const Grid& g, const float* S32, bool by_entry, const uint8_t* fix
              "Galerkin level too large for 32-bit offsets");
   // column groups of at most six: with an eight-column instance an 8-column
   // cycle on 440x64x48 took 3.2 ms, as two groups of four 2.7 ms, and twelve
-  // columns 4.1 ms against 3.4 ms as two groups of six
+  // columns 4.1 ms against 3.4 ms as two groups of six; five columns (the
+  // five load cases) 1.96 ms in the six-column instance, 1.91 ms in their own
   if (ncols <= 1) gal_apply_nc<1>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else if (ncols <= 2) gal_apply_nc<2>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else if (ncols <= 4) gal_apply_nc<4>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
+  else if (ncols == 5) gal_apply_nc<5>(g, S32, by_entry, fixed, x, y, ncols, ep, st);   // the block PCG
   else if (ncols <= 6) gal_apply_nc<6>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else if (ncols <= 8) gal_apply_nc<4>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
   else gal_apply_nc<6>(g, S32, by_entry, fixed, x, y, ncols, ep, st);
