@@ -371,7 +371,9 @@ k_apply_fast(Grid g, int zchunk, int tiles_x, int tiles_y, int ncols, FastCoef<T
   const Rsrc rfix = make_rsrc(fixed, (uint32_t)g.n_nodes);
   const Rsrc rfk = make_rsrc(fK, DO_K ? ebytes : 0u);
   const Rsrc rfm = make_rsrc(fM, DO_M ? ebytes : 0u);
-  const Rsrc rdi = make_rsrc(ep.dinv, (MODE != EPI_PLAIN && ep.dinv) ? vbytes : 0u);
+  // (a residual pass without a direction output does not read D^-1)
+  const Rsrc rdi = make_rsrc(ep.dinv, (MODE != EPI_PLAIN && ep.dinv && (MODE != EPI_RESID || ep.dnew))
+                                          ? vbytes : 0u);
   const Rsrc rb = make_rsrc((MODE == EPI_RESID || MODE == EPI_X) ? ep.b : (MODE == EPI_CHEB ? ep.r : x),
                             MODE == EPI_PLAIN ? 0u : vbytes);        // epilogue input a
   const Rsrc rxs = make_rsrc(ep.xs, (MODE == EPI_CHEB || MODE == EPI_CHEB0 || (XF && !ep.out64))
