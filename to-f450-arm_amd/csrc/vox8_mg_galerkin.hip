@@ -551,7 +551,6 @@ static void gal_apply_nc(const Grid& g, const float* S32, bool by_entry, const u
     case EPI_X: hipLaunchKernelGGL((KERN<EPI_X, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;         \
     default: hipLaunchKernelGGL((KERN<EPI_PLAIN, NC>), grid, blk, 0, st, g, ncols, per_xcd, S32, fixed, x, y, ep); break;        \
   }
-  static const int unr = getenv("VOX8_GAL_UNR") ? atoi(getenv("VOX8_GAL_UNR")) : 1;
   if (by_entry) {
     VOX8_GAL_LAUNCH(k_gal_apply_lane)
   } else {
