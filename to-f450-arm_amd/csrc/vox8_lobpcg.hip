@@ -186,8 +186,18 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
   const int npair = mm / 2;
   const int nA = npair * npair, nroles = nA + m * npair;
   // threshold cyclic Jacobi: a rotation is applied when |a_pq| exceeds
-  // eps sqrt(|a_pp a_qq|); a sweep that applies none ends the iteration
-  // (off-diagonal entries are then at the rounding level of the diagonal)
+  // eps sqrt(|a_pp a_qq|) (compared squared); a sweep that applies none ends
+  // the iteration (off-diagonal entries are then at the rounding level of the
+  // diagonal).  The rotation's tangent t = sign(d h) |h| / (|d| + sqrt(d^2 +
+  // h^2)), d = a_qq - a_pp, h = 2 a_pq, is computed in FP32 on d and h scaled
+  // by a power of two, and c = 1/sqrt(1 + t^2), s = t c in FP64: the rotation
+  // is orthogonal to FP64 rounding whatever t is, so only the angle is
+  // approximate (relative 1e-7: the entry drops to 1e-7 of itself instead of
+  // zero and the next sweep removes the rest; sweep counts are unchanged on
+  // LOBPCG-like Grams of size 12-24).  That replaces three FP64 divides and two
+  // square roots, the latency of every round.  Tangents the FP32 path would
+  // flush (|h| < 1e-30 |d|) take the FP64 formula.
+  const double eps2 = eps * eps;
   for (int sweep = 0; sweep < 40; ++sweep) {
     if (tid == 0) s_rot = 0;
     __syncthreads();
@@ -201,11 +211,20 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
         if (q < m) {
           const double apq = A[p * MMAX + q];
           const double app = A[p * MMAX + p], aqq = A[q * MMAX + q];
-          if (fabs(apq) > 1e-300 && fabs(apq) > eps * sqrt(fabs(app) * fabs(aqq))) {
-            const double tau = (aqq - app) / (2.0 * apq);
-            const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau))
-                                        : -1.0 / (-tau + sqrt(1.0 + tau * tau));
-            c = 1.0 / sqrt(1.0 + t * t);
+          if (fabs(apq) > 1e-150 && apq * apq > eps2 * fabs(app * aqq)) {
+            const double d = aqq - app, h = 2.0 * apq;
+            int ex;
+            frexp(fmax(fabs(d), fabs(h)), &ex);
+            const float ds = (float)ldexp(fabs(d), -ex), hs = (float)ldexp(fabs(h), -ex);
+            const float tf = hs * __builtin_amdgcn_rcpf(ds + __builtin_amdgcn_sqrtf(ds * ds + hs * hs));
+            double t;
+            if (tf >= 1e-30f) {
+              t = (d > 0.0 && h < 0.0) || (d < 0.0 && h > 0.0) ? -(double)tf : (double)tf;
+            } else {
+              const double tau = d / h;
+              t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+            }
+            c = rsqrt(1.0 + t * t);
             sn = t * c;
             s_rot = 1;
           }
