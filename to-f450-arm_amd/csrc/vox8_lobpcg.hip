@@ -176,19 +176,26 @@ k_gram_finish(int chunks, int m, const double* __restrict__ partials, double* __
 // every entry belongs to exactly one role, so the pass has no hazards.
 constexpr int RR_THREADS = 256;
 
+// the threshold test of the Jacobi method below: rotate (p, q) when |a_pq|
+// exceeds eps sqrt(|a_pp a_qq|), compared squared
+__device__ __forceinline__ bool jacobi_rotates(double apq, double app, double aqq, double eps2) {
+  return fabs(apq) > 1e-150 && apq * apq > eps2 * fabs(app * aqq);
+}
+
 __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
   __shared__ double rc[MMAX / 2 + 1], rs[MMAX / 2 + 1];
   __shared__ int rp[MMAX / 2 + 1], rq[MMAX / 2 + 1];
-  __shared__ int s_rot;
   const int tid = threadIdx.x, nth = blockDim.x;
   for (int e = tid; e < m * m; e += nth) V[(e / m) * MMAX + e % m] = (e / m == e % m) ? 1.0 : 0.0;
   const int mm = (m & 1) ? m + 1 : m;
   const int npair = mm / 2;
   const int nA = npair * npair, nroles = nA + m * npair;
-  // threshold cyclic Jacobi: a rotation is applied when |a_pq| exceeds
-  // eps sqrt(|a_pp a_qq|) (compared squared); a sweep that applies none ends
-  // the iteration (off-diagonal entries are then at the rounding level of the
-  // diagonal).  The rotation's tangent t = sign(d h) |h| / (|d| + sqrt(d^2 +
+  // threshold cyclic Jacobi (jacobi_rotates); the iteration ends before a
+  // sweep that would apply no rotation (off-diagonal entries are then at the
+  // rounding level of the diagonal).  Such a sweep is known before it starts:
+  // nothing changes until some pair rotates, so it rotates none exactly when
+  // no pair passes the test at its start, which all threads check at once.
+  // The rotation's tangent t = sign(d h) |h| / (|d| + sqrt(d^2 +
   // h^2)), d = a_qq - a_pp, h = 2 a_pq, is computed in FP32 on d and h scaled
   // by a power of two, and c = 1/sqrt(1 + t^2), s = t c in FP64: the rotation
   // is orthogonal to FP64 rounding whatever t is, so only the angle is
@@ -199,8 +206,12 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
   // flush (|h| < 1e-30 |d|) take the FP64 formula.
   const double eps2 = eps * eps;
   for (int sweep = 0; sweep < 40; ++sweep) {
-    if (tid == 0) s_rot = 0;
-    __syncthreads();
+    int need = 0;
+    for (int e = tid; e < m * m; e += nth) {
+      const int p = e / m, q = e - p * m;
+      if (p < q) need |= jacobi_rotates(A[p * MMAX + q], A[p * MMAX + p], A[q * MMAX + q], eps2);
+    }
+    if (!__syncthreads_or(need)) break;
     for (int round = 0; round < mm - 1; ++round) {
       if (tid < npair) {
         const int i = tid;
@@ -211,7 +222,7 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
         if (q < m) {
           const double apq = A[p * MMAX + q];
           const double app = A[p * MMAX + p], aqq = A[q * MMAX + q];
-          if (fabs(apq) > 1e-150 && apq * apq > eps2 * fabs(app * aqq)) {
+          if (jacobi_rotates(apq, app, aqq, eps2)) {
             const double d = aqq - app, h = 2.0 * apq;
             int ex;
             frexp(fmax(fabs(d), fabs(h)), &ex);
@@ -226,7 +237,6 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
             }
             c = rsqrt(1.0 + t * t);
             sn = t * c;
-            s_rot = 1;
           }
         } else {
           q = -1;                           // p is paired with the dummy index
@@ -263,9 +273,6 @@ __device__ void jacobi_eig(double* A, double* V, int m, double eps) {
       }
       __syncthreads();
     }
-    const int rot = s_rot;
-    __syncthreads();                        // every thread has read the flag before its reset
-    if (!rot) break;
   }
 }
 
@@ -334,14 +341,21 @@ __device__ bool rr_solve(int m, int ld, int k, const double* __restrict__ G, dou
   }
   __syncthreads();
   jacobi_eig(B, V, r, jeps);                                // V: eigenvectors Y (r x r)
-  if (tid == 0) {
-    for (int i = 0; i < r; ++i) { mu[i] = B[i * MMAX + i]; perm[i] = i; }
-    for (int i = 0; i < r; ++i) {                           // selection sort, descending mu
-      int b = i;
-      for (int j = i + 1; j < r; ++j)
-        if (mu[perm[j]] > mu[perm[b]]) b = j;
-      const int t = perm[i]; perm[i] = perm[b]; perm[b] = t;
+  for (int i = tid; i < r; i += nth) mu[i] = B[i * MMAX + i];
+  __syncthreads();
+  // descending mu by rank, ties by index; NaN ranks as -inf, so the ranks
+  // always form a permutation (a NaN among the k largest fails the check below)
+  for (int i = tid; i < r; i += nth) {
+    const double v = mu[i] == mu[i] ? mu[i] : -INFINITY;
+    int rank = 0;
+    for (int j = 0; j < r; ++j) {
+      const double u = mu[j] == mu[j] ? mu[j] : -INFINITY;
+      rank += u > v || (u == v && j < i);
     }
+    perm[rank] = i;
+  }
+  __syncthreads();
+  if (tid == 0) {
     bool bad = r < k;
     for (int j = 0; j < k && j < r; ++j) bad = bad || !(mu[perm[j]] > 0.0);
     s_bad = bad;
